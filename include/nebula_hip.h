@@ -1,0 +1,242 @@
+/*
+ * nebula_hip: NEBULA-HL stage two on AMD Instinct MI355X (gfx950), behind a
+ * C ABI shaped like edge-rs's `nebula_sparse` / `nebula_sparse_gpu`.
+ *
+ * Every array is a plain pointer plus a length; nothing here names a torch or
+ * HIP type, so a Rust `-sys` crate, ctypes or cgo binds it one to one.
+ *
+ * Reference interfaces replaced (edge-rs 0.4.0):
+ *   nebula_params_t    <- NebulaParams           src/sc/nebula.rs:207-258
+ *   nebula_fit_t       <- NebulaFit              src/sc/nebula.rs:301-344
+ *   nebula_sparse_hip  <- nebula_sparse_gpu      src/gpu/stage_two.rs:233-265
+ *                         (same inputs as nebula_sparse, src/sc/nebula.rs:434-445)
+ *   nh_resident_*      <- ResidentBatch upload / submit / collect
+ *                                                src/gpu/nebula_gpu.rs:244,466,605
+ *   nh_request_t       <- PmlRequest             src/gpu/nebula_gpu.rs:66
+ *   nh_reply_t         <- PmlReply               src/gpu/nebula_gpu.rs:86
+ *   NH_ERR_*           <- EdgeErrors variants on this path, src/errors.rs:13-223
+ *
+ * Threading: calls on distinct handles are independent.  One call of
+ * nebula_sparse_hip owns its device buffers and stream for its duration.
+ */
+#ifndef NEBULA_HIP_H
+#define NEBULA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes: one per EdgeErrors variant the path can raise. */
+enum {
+  NH_OK = 0,
+  NH_ERR_EMPTY_COUNTS = 1,            /* EdgeErrors::EmptyCounts */
+  NH_ERR_INVALID_ARGUMENT = 2,        /* EdgeErrors::InvalidArgument */
+  NH_ERR_MUST_BE_POSITIVE = 3,        /* EdgeErrors::MustBePositive */
+  NH_ERR_LENGTH_MISMATCH = 4,         /* EdgeErrors::LengthMismatch */
+  NH_ERR_SHAPE_MISMATCH = 5,          /* EdgeErrors::ShapeMismatch */
+  NH_ERR_MALFORMED_SPARSE = 6,        /* EdgeErrors::MalformedSparse */
+  NH_ERR_SUBJECTS_NOT_CONTIGUOUS = 7, /* EdgeErrors::SubjectsNotContiguous */
+  NH_ERR_TOO_FEW_SUBJECTS = 8,        /* EdgeErrors::TooFewSubjects */
+  NH_ERR_MISSING_INTERCEPT = 9,       /* EdgeErrors::MissingIntercept */
+  NH_ERR_NO_GENES_AFTER_FILTERING = 10, /* EdgeErrors::NoGenesAfterFiltering */
+  NH_ERR_INVALID_BOUNDS = 11,         /* EdgeErrors::InvalidBounds */
+  NH_ERR_INVALID_DISPERSION = 12,     /* EdgeErrors::InvalidDispersion */
+  NH_ERR_GPU = 13                     /* EdgeErrors::Gpu */
+};
+
+/* NEBULA method, NebulaMethod (src/sc/nebula.rs:195-205). */
+enum { NH_METHOD_LN = 0, NH_METHOD_HL = 1 };
+
+/* Convergence codes (src/sc/pml.rs:89-111, src/sc/nebula.rs:187). */
+enum {
+  NH_CONV_SUCCESS = 1,
+  NH_CONV_CRITICAL_POINT = -10,
+  NH_CONV_MAX_ITER = -20,
+  NH_CONV_SINGULAR = -25,
+  NH_CONV_NOT_FINITE = -30,
+  NH_CONV_STALLED = -40,
+  NH_CONV_OUTER_FAILED = -50,
+  NH_CONV_AT_BOUND = -60
+};
+
+/* NebulaParams (src/sc/nebula.rs:207-258); defaults from
+ * nebula_params_default() are the R package's (:260-275). */
+typedef struct {
+  double min_subject; /* min.0, lower bound on sigma^2 (1e-4) */
+  double min_cell;    /* min.1, lower bound on the NB size (1e-4) */
+  double max_subject; /* max.0 (10) */
+  double max_cell;    /* max.1 (1000) */
+  int32_t method;     /* NH_METHOD_LN (default) or NH_METHOD_HL */
+  int32_t reml;       /* 0 (default) */
+  double cutoff_cell; /* 20 */
+  double kappa;       /* 800 */
+  double cpc;         /* 0.005 */
+  uint64_t mincp;     /* 5 */
+  uint64_t min_subjects; /* 0 */
+  double eps;         /* 1e-6 */
+} nebula_params_t;
+
+/* NebulaFit (src/sc/nebula.rs:301-344).  Arrays are owned by the library and
+ * released with nebula_fit_free.  Row-major over kept genes. */
+typedef struct {
+  size_t n_genes;                  /* genes that survived the filter */
+  size_t n_coef;
+  double* coefficients;            /* n_genes * n_coef */
+  double* covariance;              /* n_genes * n_coef(n_coef+1)/2, packed
+                                      upper triangle: (i,j), i<=j, at j(j+1)/2+i */
+  double* se;                      /* n_genes * n_coef */
+  double* subject_overdispersion;  /* n_genes: sigma^2 */
+  double* cell_overdispersion;     /* n_genes: phi = 1 / NB size */
+  int32_t* convergence;            /* n_genes */
+  uint8_t* sigma_at_bound;         /* n_genes, 0/1 */
+  uint64_t* gene_index;            /* n_genes, zero-based input rows */
+} nebula_fit_t;
+
+/* Which stage-two engine nebula_sparse_hip_ex runs. */
+enum {
+  NH_ENGINE_PLAIN = 0,   /* host BOBYQA, one fit launch per search round */
+  NH_ENGINE_BATCHED = 1  /* persistent work-queue kernel, search on device */
+};
+
+/* Options beyond the reference's arguments (no counterpart in edge-rs). */
+typedef struct {
+  int32_t engine;        /* NH_ENGINE_* (default NH_ENGINE_BATCHED) */
+  int32_t learned;       /* 1: MLP warm start for the search (needs weights) */
+  const float* mlp_weights; /* packed MLP parameters, see nh_mlp_param_count */
+  int32_t wg_size;       /* threads per fit workgroup, 0 = autotuned */
+  int32_t device;        /* HIP device ordinal */
+  int32_t host_threads;  /* 0 = hardware concurrency */
+} nebula_hip_options_t;
+
+/* Per-phase timings and counters of one call (EDGE_RS_GPU_TIMING's phases,
+ * src/gpu/stage_two.rs:90-194). */
+typedef struct {
+  double t_setup, t_stage_one, t_stage_two, t_stage_three, t_total; /* seconds */
+  double t_upload;
+  uint64_t genes_searched;   /* genes entering stage two */
+  uint64_t evaluations;      /* objective evaluations, all genes */
+  uint64_t rounds;           /* launches of the stage-two fit kernel */
+  uint64_t sweeps_f32;       /* passes over a gene's cells, FP32 */
+  uint64_t sweeps_f64;       /* passes over a gene's cells, FP64 */
+  uint64_t cell_sweeps;      /* sum over sweeps of the gene's cell count */
+  uint64_t nnz_sweeps;       /* sum over sweeps of the gene's positive counts */
+  uint64_t warm_fallbacks;   /* learned starts rejected by the log-lik rule */
+  double fit_kernel_ms;      /* device time of the stage-two kernels (events) */
+  uint64_t fit_kernel_launches;
+} nebula_run_stats_t;
+
+void nebula_params_default(nebula_params_t* params);
+void nebula_hip_options_default(nebula_hip_options_t* options);
+
+/* nebula_sparse_gpu's call shape: CSR counts over (n_genes, n_cells) with u32
+ * indptr (n_genes + 1) and indices, f64 values; subject id per cell (each
+ * subject one contiguous run); design n_cells * n_coef row-major with exactly
+ * one constant column; offset per cell on the linear scale (NULL for ones);
+ * params NULL for defaults.  Returns NH_OK or an NH_ERR_* code with a UTF-8
+ * message in err.  n_coef above 8 and reml are rejected, as on the reference
+ * GPU path (src/gpu/stage_two.rs:242-246, src/gpu/pml_kernel.rs:1208-1212). */
+int nebula_sparse_hip(size_t n_genes, size_t n_cells, const uint32_t* indptr,
+                      const uint32_t* indices, const double* data,
+                      const uint64_t* subject_id, const double* design,
+                      size_t n_coef, const double* offset,
+                      const nebula_params_t* params, nebula_fit_t* out,
+                      char* err, size_t err_len);
+
+/* Same, with engine options and run statistics (either may be NULL). */
+int nebula_sparse_hip_ex(size_t n_genes, size_t n_cells, const uint32_t* indptr,
+                         const uint32_t* indices, const double* data,
+                         const uint64_t* subject_id, const double* design,
+                         size_t n_coef, const double* offset,
+                         const nebula_params_t* params,
+                         const nebula_hip_options_t* options,
+                         nebula_fit_t* out, nebula_run_stats_t* stats,
+                         char* err, size_t err_len);
+
+void nebula_fit_free(nebula_fit_t* fit);
+
+/* ---------------------------------------------------------------------------
+ * Request-level batch API: one penalised fit (opt_pml) per request, all on the
+ * device.  Mirrors ResidentBatch (src/gpu/nebula_gpu.rs:195-648).
+ * ------------------------------------------------------------------------- */
+
+typedef struct nh_resident nh_resident_t;
+
+/* One penalised fit at fixed variance components, PmlRequest
+ * (src/gpu/nebula_gpu.rs:66-84). */
+typedef struct {
+  uint32_t gene;         /* resident gene index */
+  uint32_t ord;          /* Laplace order 1..3 */
+  double subject;        /* nebula's sigma[0] = log(1 + sigma^2) */
+  double cell;           /* NB size gamma */
+  const double* beta_init;  /* n_coef */
+  const double* log_w_init; /* n_subjects, or NULL for zeros */
+} nh_request_t;
+
+/* PmlReply (src/gpu/nebula_gpu.rs:86-130) plus PmlResult's fields
+ * (src/sc/pml.rs:288-330).  Arrays are caller-allocated views. */
+typedef struct {
+  double log_likelihood;
+  double log_likelihood_prev;
+  double log_det;
+  double second_order;
+  int32_t convergence;
+  int32_t iterations;
+  int32_t backtracks;
+  int32_t finish_steps;   /* FP64 Newton steps after the FP32 solve */
+  double* beta;           /* n_coef */
+  double* log_w;          /* n_subjects */
+  double* information;    /* n_coef * n_coef */
+} nh_reply_t;
+
+/* Precision policy of a batch solve. */
+enum {
+  NH_SOLVE_F32_FINISH = 0, /* FP32 Newton, then FP64 finish (stage two) */
+  NH_SOLVE_F64 = 1         /* opt_pml in FP64 on the device (stage three) */
+};
+
+/* Uploads the shared design (n_cells * n_coef, already centred, f64), log
+ * offsets, subject boundaries (n_subjects + 1) and per-gene positive counts
+ * (CSR over genes: gene_ptr n_genes+1, cells, counts) once. */
+int nh_resident_upload(size_t n_cells, size_t n_coef, size_t n_subjects,
+                       const double* design, const double* log_offset,
+                       const uint64_t* subject_start, size_t n_genes,
+                       const uint64_t* gene_ptr, const uint32_t* cells,
+                       const double* counts, int32_t device,
+                       nh_resident_t** out, char* err, size_t err_len);
+
+/* Solves n_req requests (submit + collect).  eps is nebula's absolute
+ * stopping tolerance; mode is NH_SOLVE_*. */
+int nh_resident_solve(nh_resident_t* r, const nh_request_t* requests,
+                      size_t n_req, double eps, int32_t mode,
+                      nh_reply_t* replies, char* err, size_t err_len);
+
+void nh_resident_free(nh_resident_t* r);
+
+/* BOBYQA as the device and the host search run it (NLopt 2.7.1 semantics).
+ * Minimises a callback; trace receives every evaluated point (n values) and
+ * its value, row by row (up to max_trace rows).  Returns the NLopt status. */
+typedef double (*nh_objective_fn)(const double* x, void* ctx);
+int nh_bobyqa(int n, const double* x0, const double* lower, const double* upper,
+              double xtol_rel, int maxeval, nh_objective_fn f, void* ctx,
+              double* x_out, double* f_out, int* evals_out);
+
+/* The same stepper run on the device, objective = a built-in test function
+ * (0 rosen, 1 corner, 2 ripple, 3 valley, 4 chain3) evaluated on the device.
+ * trace: max_trace * (n + 1) doubles.  Returns the NLopt status or a negative
+ * NH_ERR_* on a device error (then err holds the message). */
+int nh_bobyqa_device_trace(int problem, int n, const double* x0,
+                           const double* lower, const double* upper,
+                           double* trace, int max_trace, int* evals_out,
+                           char* err, size_t err_len);
+
+/* Number of float parameters of the warm-start MLP (12 -> 64 -> 64 -> 64 -> 2). */
+size_t nh_mlp_param_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NEBULA_HIP_H */

@@ -1,0 +1,243 @@
+"""ctypes view of the CPU f64 oracle (oracle/lib/liboracle.so).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's CPU
+baseline import this module, and only as the checker.  Build with
+``make -C oracle`` (``__graft_entry__.build()`` does it).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "liboracle.so")
+
+_dp = C.POINTER(C.c_double)
+_u64p = C.POINTER(C.c_uint64)
+_u32p = C.POINTER(C.c_uint32)
+_ip = C.POINTER(C.c_int)
+
+
+class NebulaParams(C.Structure):
+    """nebula_params_t (include/nebula_hip.h), NebulaParams in edge-rs."""
+    _fields_ = [("min_subject", C.c_double), ("min_cell", C.c_double),
+                ("max_subject", C.c_double), ("max_cell", C.c_double),
+                ("method", C.c_int32), ("reml", C.c_int32),
+                ("cutoff_cell", C.c_double), ("kappa", C.c_double),
+                ("cpc", C.c_double), ("mincp", C.c_uint64),
+                ("min_subjects", C.c_uint64), ("eps", C.c_double)]
+
+
+class NebulaFitC(C.Structure):
+    _fields_ = [("n_genes", C.c_size_t), ("n_coef", C.c_size_t),
+                ("coefficients", _dp), ("covariance", _dp), ("se", _dp),
+                ("subject_overdispersion", _dp), ("cell_overdispersion", _dp),
+                ("convergence", C.POINTER(C.c_int32)),
+                ("sigma_at_bound", C.POINTER(C.c_uint8)),
+                ("gene_index", _u64p)]
+
+
+def params(method="LN", **kw):
+    p = NebulaParams(1e-4, 1e-4, 10.0, 1000.0, 0, 0, 20.0, 800.0, 0.005, 5, 0, 1e-6)
+    p.method = 1 if method == "HL" else 0
+    for k, v in kw.items():
+        if k == "min":
+            p.min_subject, p.min_cell = v
+        elif k == "max":
+            p.max_subject, p.max_cell = v
+        else:
+            setattr(p, k, v)
+    return p
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing at {LIB_PATH}; run make -C oracle")
+        L = C.CDLL(LIB_PATH)
+        L.oracle_nebula_sparse.restype = C.c_int
+        L.oracle_nebula_sparse.argtypes = [C.c_size_t, C.c_size_t, _u32p, _u32p, _dp, _u64p,
+                                           _dp, C.c_size_t, _dp, C.POINTER(NebulaParams),
+                                           C.c_int, C.POINTER(NebulaFitC), _dp, _dp,
+                                           C.c_char_p, C.c_size_t]
+        L.oracle_fit_free.argtypes = [C.POINTER(NebulaFitC)]
+        L.oracle_opt_pml.restype = C.c_int
+        L.oracle_opt_pml.argtypes = [_dp, _dp, C.c_size_t, C.c_size_t, _dp, _u64p, C.c_size_t,
+                                     _u64p, C.c_size_t, _dp, _dp, _dp, C.c_double, C.c_double,
+                                     C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, _dp, _dp,
+                                     _dp, _dp, _ip, C.c_char_p, C.c_size_t]
+        L.oracle_pml_ll_grad.restype = C.c_int
+        L.oracle_pml_ll_grad.argtypes = [_dp, _dp, C.c_size_t, C.c_size_t, _dp, _u64p,
+                                         C.c_size_t, _u64p, C.c_size_t, _dp, _dp, _dp,
+                                         C.c_double, C.c_double, _dp, _dp]
+        L.oracle_ptmg.restype = C.c_int
+        L.oracle_ptmg.argtypes = [_dp, _dp, C.c_size_t, C.c_size_t, _dp, _u64p, C.c_size_t,
+                                  _dp, _u64p, C.c_size_t, _dp, _dp, _dp, _dp]
+        L.oracle_special.argtypes = [C.c_int, _dp, _dp, C.c_size_t]
+        L.oracle_min_eigenvalue.restype = C.c_double
+        L.oracle_min_eigenvalue.argtypes = [_dp, C.c_size_t]
+        L.oracle_ldlt_solve.argtypes = [_dp, _dp, C.c_size_t]
+        L.oracle_determinant.restype = C.c_double
+        L.oracle_determinant.argtypes = [_dp, C.c_size_t]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _u64(a):
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    return a, a.ctypes.data_as(_u64p)
+
+
+def _u32(a):
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    return a, a.ctypes.data_as(_u32p)
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+def opt_pml(design, offset, counts, cells, subject_start, subject_total, beta_init,
+            subject, cell, reml=False, eps=1e-6, ord=1, log_w_init=None, max_iter=0,
+            nbm=False):
+    """oracle opt_pml (src/sc/pml.rs:468) on one gene; returns a dict."""
+    design = np.asarray(design, dtype=np.float64)
+    n_cells = len(offset)
+    nb = design.size // n_cells
+    k = len(subject_start) - 1
+    keep = []
+    args = []
+    for a in (design, offset, counts):
+        x, p = _d(a)
+        keep.append(x)
+        args.append(p)
+    c, cp = _u64(cells)
+    s, sp = _u64(subject_start)
+    t, tp = _d(subject_total)
+    b, bp = _d(beta_init)
+    lw = None
+    lwp = None
+    if log_w_init is not None:
+        lw, lwp = _d(log_w_init)
+    beta = np.zeros(nb)
+    log_w = np.zeros(k)
+    info = np.zeros(nb * nb)
+    sc = np.zeros(4)
+    ints = np.zeros(3, dtype=np.int32)
+    err = C.create_string_buffer(512)
+    rc = lib().oracle_opt_pml(args[0], args[1], n_cells, nb, args[2], cp, len(c), sp, k, tp,
+                              bp, lwp, subject, cell, int(reml), eps, ord, max_iter, int(nbm),
+                              beta.ctypes.data_as(_dp), log_w.ctypes.data_as(_dp),
+                              info.ctypes.data_as(_dp), sc.ctypes.data_as(_dp),
+                              ints.ctypes.data_as(_ip), err, 512)
+    if rc != 0:
+        raise OracleError(rc, err.value.decode())
+    return dict(beta=beta, log_w=log_w, information=info, log_likelihood=sc[0],
+                log_likelihood_prev=sc[1], log_det=sc[2], second_order=sc[3],
+                convergence=int(ints[0]), iterations=int(ints[1]), backtracks=int(ints[2]))
+
+
+def pml_ll_grad(design, offset, counts, cells, subject_start, subject_total, beta, log_w,
+                subject, cell):
+    design = np.asarray(design, dtype=np.float64)
+    n_cells = len(offset)
+    nb = design.size // n_cells
+    k = len(subject_start) - 1
+    d, dp = _d(design)
+    o, op = _d(offset)
+    y, yp = _d(counts)
+    c, cp = _u64(cells)
+    s, sp = _u64(subject_start)
+    t, tp = _d(subject_total)
+    b, bp = _d(beta)
+    w, wp = _d(log_w)
+    obj = C.c_double()
+    grad = np.zeros(nb + k)
+    rc = lib().oracle_pml_ll_grad(dp, op, n_cells, nb, yp, cp, len(c), sp, k, tp, bp, wp,
+                                  subject, cell, C.byref(obj), grad.ctypes.data_as(_dp))
+    if rc:
+        raise OracleError(rc, "pml_ll_grad")
+    return obj.value, grad
+
+
+def ptmg(design, log_offset, counts, cells, totals, fid, params, hessian=True):
+    """Stage-one marginal NLL, gradient and Hessian (src/sc/ptmg.rs:585,634)."""
+    design = np.asarray(design, dtype=np.float64)
+    n = len(log_offset)
+    nb = design.size // n
+    d, dp = _d(design)
+    o, op = _d(log_offset)
+    y, yp = _d(counts)
+    c, cp = _u64(cells)
+    t, tp = _d(totals)
+    f, fp = _u64(fid)
+    x, xp = _d(params)
+    v = C.c_double()
+    g = np.zeros(nb + 2)
+    h = np.zeros((nb + 2) ** 2) if hessian else None
+    lib().oracle_ptmg(dp, op, n, nb, yp, cp, len(c), tp, fp, len(f) - 1, xp, C.byref(v),
+                      g.ctypes.data_as(_dp), h.ctypes.data_as(_dp) if hessian else None)
+    return v.value, g, h
+
+
+def special(which, x):
+    x, xp = _d(np.atleast_1d(x))
+    y = np.zeros_like(x)
+    lib().oracle_special({"lgamma": 0, "digamma": 1, "trigamma": 2}[which], xp,
+                         y.ctypes.data_as(_dp), len(x))
+    return y
+
+
+def nebula_sparse(indptr, indices, data, n_cells, subject_id, design, n_coef, offset=None,
+                  prm=None, threads=0, with_stage_two=False):
+    """oracle nebula_sparse (src/sc/nebula.rs:434) -> dict of numpy arrays."""
+    n_genes = len(indptr) - 1
+    ip, ipp = _u32(indptr)
+    ix, ixp = _u32(indices)
+    dv, dvp = _d(data)
+    sid, sidp = _u64(subject_id)
+    de, dep = _d(design)
+    off = None
+    offp = None
+    if offset is not None:
+        off, offp = _d(offset)
+    fit = NebulaFitC()
+    info = np.zeros(8 * n_genes) if with_stage_two else None
+    t2 = C.c_double(0.0)
+    err = C.create_string_buffer(1024)
+    rc = lib().oracle_nebula_sparse(n_genes, n_cells, ipp, ixp, dvp, sidp, dep, n_coef, offp,
+                                    C.byref(prm) if prm is not None else None, threads,
+                                    C.byref(fit),
+                                    info.ctypes.data_as(_dp) if info is not None else None,
+                                    C.byref(t2), err, 1024)
+    if rc != 0:
+        raise OracleError(rc, err.value.decode())
+    g = fit.n_genes
+    p = fit.n_coef
+    pk = p * (p + 1) // 2
+    out = dict(
+        coefficients=np.ctypeslib.as_array(fit.coefficients, (g * p,)).reshape(g, p).copy(),
+        covariance=np.ctypeslib.as_array(fit.covariance, (g * pk,)).reshape(g, pk).copy(),
+        se=np.ctypeslib.as_array(fit.se, (g * p,)).reshape(g, p).copy(),
+        subject_overdispersion=np.ctypeslib.as_array(fit.subject_overdispersion, (g,)).copy(),
+        cell_overdispersion=np.ctypeslib.as_array(fit.cell_overdispersion, (g,)).copy(),
+        convergence=np.ctypeslib.as_array(fit.convergence, (g,)).copy(),
+        sigma_at_bound=np.ctypeslib.as_array(fit.sigma_at_bound, (g,)).astype(bool),
+        gene_index=np.ctypeslib.as_array(fit.gene_index, (g,)).copy(),
+        t_stage_two=t2.value,
+    )
+    if info is not None:
+        out["stage_two"] = info[: 8 * g].reshape(g, 8).copy()
+    lib().oracle_fit_free(C.byref(fit))
+    return out

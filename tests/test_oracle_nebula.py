@@ -1,0 +1,174 @@
+"""End-to-end pins of the oracle's nebula_sparse against R nebula 1.5.8/1.5.9
+goldens committed in the reference (src/sc/nebula.rs:2457-3187 tolerances for
+the 8-gene set; tests/e2e_nebula.rs:66-254 for the sc_* sets), restricted to
+the HL path this build covers."""
+import numpy as np
+import pytest
+
+import oracle
+from fixtures import (SC_DATASETS, assert_close, dense_to_csr, golden8, load_sc, read_table,
+                      repack_r_cov)
+
+
+def run_golden8(design=None, n_coef=3, counts=None, **kw):
+    c, subject, d, offset = golden8()
+    if counts is not None:
+        c = counts
+    if design is not None:
+        d = design
+    ip, ix, dv = dense_to_csr(c)
+    prm = oracle.params(method=kw.pop("method", "LN"), cutoff_cell=0.0, **kw)
+    return oracle.nebula_sparse(ip, ix, dv, 150, subject, d, n_coef, offset, prm)
+
+
+def test_golden8_matches_r():
+    fit = run_golden8()
+    _, exp = read_table("nebula_expected.csv")
+    _, cov = read_table("nebula_covariance.csv")
+    assert list(fit["gene_index"]) == list(range(8))
+    assert_close(fit["coefficients"], exp[:, 0:3], 5e-6, label="coef")
+    assert_close(fit["se"], exp[:, 3:6], 1e-5, label="se")
+    assert_close(fit["subject_overdispersion"], exp[:, 10], 1e-5, label="subject")
+    assert_close(fit["cell_overdispersion"], exp[:, 11], 1e-5, label="cell")
+    want_cov = np.array([[r[0], r[1], r[3], r[2], r[4], r[5]] for r in cov])
+    assert_close(fit["covariance"], want_cov, 1e-5, label="cov")
+    for g in range(8):
+        if exp[g, 12] == 1:
+            assert fit["convergence"][g] in (1, -10)
+        else:
+            assert fit["convergence"][g] == exp[g, 12]
+    # genes 2, 4 and 7 (one-based) sit on the sigma^2 floor (nebula.rs:2676-2720)
+    assert list(np.nonzero(fit["sigma_at_bound"])[0]) == [1, 3, 6]
+
+
+def test_golden8_hl_equals_ln_below_thirty_cells_per_subject():
+    a = run_golden8()
+    b = run_golden8(method="HL")
+    assert np.array_equal(a["coefficients"], b["coefficients"])
+
+
+def test_golden8_two_column_design():
+    c, subject, d, offset = golden8()
+    fit = run_golden8(design=d[:, :2].copy(), n_coef=2)
+    want = np.array([
+        [-3.83069111381507010e+00, 4.67477914516942039e-01, 1.52851759868728232e-01,
+         3.05703519737456464e-01, 1.19930138981593701e-01, 2.50509063784787156e-01],
+        [-5.22622507034590900e+00, 2.86854056278882219e-01, 1.44924959049115953e-01,
+         2.89849918098231907e-01, 1.02875112505358357e-01, 2.46112537450009855e-01],
+        [-2.79850766191427391e+00, 3.82576141900636657e-01, 1.46513262535639927e-01,
+         2.93026525071279853e-01, 1.09839244633263694e-01, 2.79001107949189975e-01],
+        [-5.78859782201794282e+00, 1.39871216066227488e-01, 1.13748802726646378e-01,
+         2.27497605453292728e-01, 5.63655105828342259e-02, 1.68916319758526501e-01],
+        [-3.18904397195006339e+00, 4.63470376690180819e-01, 1.71383246709902126e-01,
+         3.42766493419804308e-01, 1.51708497662862124e-01, 2.53029183723021411e-01],
+        [-4.30946341634605812e+00, 2.84795644398078140e-01, 1.12103980708383288e-01,
+         2.24207961416766549e-01, 6.00916997253425900e-02, 2.56315004910657396e-01],
+        [-2.27100390853884093e+00, 3.57947777093715835e-01, 1.21567297349690767e-01,
+         2.43134594699381507e-01, 7.29528753821947207e-02, 3.03612425865116442e-01],
+        [-4.71507442768750007e+00, 5.85766182363788412e-01, 1.48580330507162378e-01,
+         2.97160661014324701e-01, 1.10500262941949104e-01, 2.60169207789380963e-01]])
+    assert_close(fit["coefficients"], want[:, 0:2], 5e-6, label="coef")
+    assert_close(fit["se"], want[:, 2:4], 1e-5, label="se")
+    assert_close(fit["subject_overdispersion"], want[:, 4], 1e-5, label="subject")
+    assert_close(fit["cell_overdispersion"], want[:, 5], 1e-5, label="cell")
+
+
+def test_golden8_subject_with_all_zero_counts():
+    c, subject, d, offset = golden8()
+    c = c.copy()
+    c[0, :25] = 0.0
+    fit = run_golden8(counts=c)
+    assert_close(fit["coefficients"][0], [-4.64175020355696155e+00, 2.11998305680204924e+00,
+                                          1.63852281831067770e+00], 5e-6)
+    assert_close(fit["se"][0], [4.24684864629024383e-01, 1.17140823143936190e+00,
+                                8.69775089819186276e-01], 1e-5)
+    assert_close(fit["subject_overdispersion"][0], 7.23281548565090415e-01, 1e-5)
+    assert_close(fit["cell_overdispersion"][0], 2.57684232626537890e-01, 1e-5)
+
+
+def test_gene_filter():
+    c, subject, d, offset = golden8()
+    three = np.zeros(150)
+    three[[3, 40, 90]] = 1
+    six_a = np.zeros(150)
+    six_a[[2, 5, 8, 11, 14, 17]] = 1
+    six_b = np.zeros(150)
+    six_b[:6] = 1
+    fit = run_golden8(counts=np.vstack([c, three, six_a, six_b]))
+    assert list(fit["gene_index"]) == [0, 1, 2, 3, 4, 5, 6, 7, 9, 10]
+    fit = run_golden8(cpc=10.0)
+    assert list(fit["gene_index"]) == [0, 2, 4, 5, 6]
+
+
+def test_error_codes():
+    c, subject, d, offset = golden8()
+    ip, ix, dv = dense_to_csr(c)
+    prm = oracle.params(cutoff_cell=0.0)
+    bad_sub = subject.copy()
+    bad_sub[149] = subject[0]
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.nebula_sparse(ip, ix, dv, 150, bad_sub, d, 3, offset, prm)
+    assert e.value.code == 7
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.nebula_sparse(ip, ix, dv, 150, np.ones(150, dtype=np.uint64), d, 3, offset, prm)
+    assert e.value.code == 8
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.nebula_sparse(ip, ix, dv, 150, subject, d[:, 1:].copy(), 2, offset, prm)
+    assert e.value.code == 9
+    bad_off = offset.copy()
+    bad_off[3] = 0
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.nebula_sparse(ip, ix, dv, 150, subject, d, 3, bad_off, prm)
+    assert e.value.code == 3
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.nebula_sparse(ip, ix, dv, 150, subject, d, 3, offset,
+                             oracle.params(cpc=1e6, cutoff_cell=0.0))
+    assert e.value.code == 10
+
+
+# Per-dataset gates (tests/e2e_nebula.rs:66-254): coef, se, cov, subject, cell.
+BASE = dict(coef=(3e-5, 1e-9), se=(1e-4, 0.0), cov=(1e-3, 1e-12), subject=(2e-4, 1e-6),
+            cell=(1e-4, 0.0))
+LOOSER = {"sc_high": dict(coef=(3e-4, 1e-9), subject=(2e-3, 1e-6), cov=(2.5e-2, 1e-12))}
+
+
+def check_sc(tag, fit):
+    d = load_sc(tag)
+    nh, neb = d["expected"]
+    col = {h: i for i, h in enumerate(nh)}
+    n_coef = d["n_coef"]
+    tol = dict(BASE, **LOOSER.get(tag, {}))
+    assert list(fit["gene_index"]) == list(neb[:, col["gene_id"]].astype(int) - 1)
+    names = [h[len("logFC_"):] for h in nh if h.startswith("logFC_")]
+    want_coef = np.column_stack([neb[:, col["logFC_" + n]] for n in names])
+    want_se = np.column_stack([neb[:, col["se_" + n]] for n in names])
+    pure = neb[:, col["algorithm"]] != 2
+    assert_close(fit["coefficients"][pure], want_coef[pure], *tol["coef"], label=f"{tag}/coef")
+    assert_close(fit["se"][pure], want_se[pure], *tol["se"], label=f"{tag}/se")
+    assert_close(fit["subject_overdispersion"][pure], neb[pure, col["Subject"]], *tol["subject"],
+                 label=f"{tag}/subject")
+    assert_close(fit["cell_overdispersion"][pure], neb[pure, col["Cell"]], *tol["cell"],
+                 label=f"{tag}/cell")
+    want_cov = np.array([repack_r_cov(r, n_coef) for r in d["covariance"]])
+    assert_close(fit["covariance"][pure], want_cov[pure], *tol["cov"], label=f"{tag}/cov")
+    for g, w in enumerate(neb[:, col["convergence"]].astype(int)):
+        got = int(fit["convergence"][g])
+        if w in (1, -10) or (w == -50 and got in (1, -10)):
+            assert got in (1, -10), (tag, g, got, w)
+        else:
+            assert got == w, (tag, g, got, w)
+
+
+def run_sc(tag, runner):
+    d = load_sc(tag)
+    ip, ix, dv = dense_to_csr(d["counts"])
+    return runner(ip, ix, dv, d)
+
+
+@pytest.mark.parametrize("tag", list(SC_DATASETS))
+def test_sc_dataset_matches_r(tag):
+    def runner(ip, ix, dv, d):
+        prm = oracle.params(method=d["method"])
+        return oracle.nebula_sparse(ip, ix, dv, d["counts"].shape[1], d["subject"], d["design"],
+                                    d["n_coef"], d["offset"], prm)
+    check_sc(tag, run_sc(tag, runner))
