@@ -1,0 +1,200 @@
+// C ABI: request-level batch API, BOBYQA entry points and defaults.
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/nebula_hip.h"
+#include "bobyqa.hpp"
+#include "resident.hpp"
+
+struct nh_resident {
+  nh::Resident r;
+};
+
+namespace {
+void put_err(char* err, size_t len, const std::string& m) {
+  if (err && len) {
+    std::strncpy(err, m.c_str(), len - 1);
+    err[len - 1] = 0;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+void nebula_params_default(nebula_params_t* p) {
+  p->min_subject = 1e-4;
+  p->min_cell = 1e-4;
+  p->max_subject = 10.0;
+  p->max_cell = 1000.0;
+  p->method = NH_METHOD_LN;
+  p->reml = 0;
+  p->cutoff_cell = 20.0;
+  p->kappa = 800.0;
+  p->cpc = 0.005;
+  p->mincp = 5;
+  p->min_subjects = 0;
+  p->eps = 1e-6;
+}
+
+void nebula_hip_options_default(nebula_hip_options_t* o) {
+  o->engine = NH_ENGINE_BATCHED;
+  o->learned = 0;
+  o->mlp_weights = nullptr;
+  o->wg_size = 0;
+  o->device = 0;
+  o->host_threads = 0;
+}
+
+int nh_resident_upload(size_t n_cells, size_t n_coef, size_t n_subjects, const double* design,
+                       const double* log_offset, const uint64_t* subject_start, size_t n_genes,
+                       const uint64_t* gene_ptr, const uint32_t* cells, const double* counts,
+                       int32_t device, nh_resident_t** out, char* err, size_t err_len) {
+  if (!out || !design || !log_offset || !subject_start || !gene_ptr) {
+    put_err(err, err_len, "null argument");
+    return NH_ERR_INVALID_ARGUMENT;
+  }
+  if (n_coef < 1 || n_coef > 8) {
+    put_err(err, err_len, "the device fit takes 1 to 8 design columns");
+    return NH_ERR_INVALID_ARGUMENT;
+  }
+  if (n_subjects < 1 || subject_start[0] != 0 || subject_start[n_subjects] != n_cells) {
+    put_err(err, err_len, "subject_start must run from 0 to n_cells");
+    return NH_ERR_INVALID_ARGUMENT;
+  }
+  for (size_t s = 0; s < n_subjects; ++s)
+    if (subject_start[s] >= subject_start[s + 1]) {
+      put_err(err, err_len, "empty subject block");
+      return NH_ERR_SUBJECTS_NOT_CONTIGUOUS;
+    }
+  for (size_t g = 0; g < n_genes; ++g)
+    for (uint64_t i = gene_ptr[g]; i < gene_ptr[g + 1]; ++i)
+      if (cells[i] >= n_cells || (i > gene_ptr[g] && cells[i] <= cells[i - 1])) {
+        put_err(err, err_len, "cells must be in range and increasing within a gene");
+        return NH_ERR_MALFORMED_SPARSE;
+      }
+  nh_resident* h = new (std::nothrow) nh_resident;
+  if (!h) {
+    put_err(err, err_len, "out of host memory");
+    return NH_ERR_GPU;
+  }
+  try {
+    h->r.upload(n_cells, n_coef, n_subjects, design, log_offset, subject_start, n_genes, gene_ptr,
+                cells, counts, device);
+  } catch (const std::exception& e) {
+    delete h;
+    put_err(err, err_len, e.what());
+    return NH_ERR_GPU;
+  }
+  *out = h;
+  return NH_OK;
+}
+
+int nh_resident_solve(nh_resident_t* h, const nh_request_t* requests, size_t n_req, double eps,
+                      int32_t mode, nh_reply_t* replies, char* err, size_t err_len) {
+  if (!h) {
+    put_err(err, err_len, "null handle");
+    return NH_ERR_INVALID_ARGUMENT;
+  }
+  nh::Resident& r = h->r;
+  const size_t nb = r.nb, k = r.k;
+  std::vector<nh::DevRequest> req(n_req);
+  std::vector<double> bi(n_req * nb), lwi;
+  bool any_lw = false;
+  for (size_t q = 0; q < n_req; ++q) any_lw = any_lw || requests[q].log_w_init;
+  if (any_lw) lwi.assign(n_req * k, 0.0);
+  for (size_t q = 0; q < n_req; ++q) {
+    const nh_request_t& x = requests[q];
+    if (x.gene >= r.n_genes || x.ord < 1 || x.ord > 3 || !x.beta_init) {
+      put_err(err, err_len, "invalid request " + std::to_string(q));
+      return NH_ERR_INVALID_ARGUMENT;
+    }
+    const double exps = std::exp(x.subject);
+    if (!(std::isfinite(exps) && exps > 1.0)) {
+      put_err(err, err_len, "subject must be finite and strictly positive");
+      return NH_ERR_INVALID_ARGUMENT;
+    }
+    if (!(std::isfinite(x.cell) && x.cell > 0.0)) {
+      put_err(err, err_len, "invalid cell dispersion");
+      return NH_ERR_INVALID_DISPERSION;
+    }
+    req[q] = nh::DevRequest{x.gene, x.ord, x.subject, x.cell};
+    for (size_t j = 0; j < nb; ++j) bi[q * nb + j] = x.beta_init[j];
+    if (any_lw && x.log_w_init)
+      for (size_t s = 0; s < k; ++s) lwi[q * k + s] = x.log_w_init[s];
+  }
+  std::vector<nh::DevReply> out;
+  std::vector<double> ob, olw, oinfo;
+  try {
+    r.solve(req, bi, any_lw ? &lwi : nullptr, eps, mode, 8, out, ob, olw, &oinfo);
+  } catch (const std::exception& e) {
+    put_err(err, err_len, e.what());
+    return NH_ERR_GPU;
+  }
+  for (size_t q = 0; q < n_req; ++q) {
+    nh_reply_t& o = replies[q];
+    o.log_likelihood = out[q].ll;
+    o.log_likelihood_prev = out[q].ll_prev;
+    o.log_det = out[q].log_det;
+    o.second_order = out[q].second;
+    o.iterations = out[q].iterations;
+    o.backtracks = out[q].backtracks;
+    o.finish_steps = out[q].finish_steps;
+    o.convergence = 0;
+    if (o.beta) std::memcpy(o.beta, &ob[q * nb], nb * sizeof(double));
+    if (o.log_w) std::memcpy(o.log_w, &olw[q * k], k * sizeof(double));
+    if (o.information) std::memcpy(o.information, &oinfo[q * nb * nb], nb * nb * sizeof(double));
+  }
+  return NH_OK;
+}
+
+void nh_resident_free(nh_resident_t* h) { delete h; }
+
+int nh_bobyqa(int n, const double* x0, const double* lower, const double* upper, double xtol_rel,
+              int maxeval, nh_objective_fn f, void* ctx, double* x_out, double* f_out,
+              int* evals_out) {
+  if (n < 1 || n > 3) return nh::BOBYQA_BAD_BOX;
+  nh::Bobyqa<3> b;
+  b.init(n, x0, lower, upper, xtol_rel, maxeval);
+  if (b.status == nh::BOBYQA_BAD_BOX) return b.status;
+  while (b.want) b.tell(f(b.xask, ctx));
+  for (int i = 0; i < n; ++i) x_out[i] = b.xbest_out[i];
+  *f_out = b.fbest_out;
+  *evals_out = b.nevals;
+  return b.status;
+}
+
+int nh_bobyqa_device_trace(int problem, int n, const double* x0, const double* lower,
+                           const double* upper, double* trace, int max_trace, int* evals_out,
+                           char* err, size_t err_len) {
+  if (n < 1 || n > 3 || max_trace < 0) {
+    put_err(err, err_len, "invalid arguments");
+    return -100 - NH_ERR_INVALID_ARGUMENT;
+  }
+  try {
+    nh::DevBuf<double> dx0, dlo, dhi, dtr;
+    nh::DevBuf<int> dres;
+    hipStream_t st = nullptr;
+    dx0.upload(x0, n, st);
+    dlo.upload(lower, n, st);
+    dhi.upload(upper, n, st);
+    dtr.alloc((size_t)max_trace * (n + 1));
+    dres.alloc(2);
+    NH_HIP(nh::launch_bobyqa_trace(problem, n, dx0.p, dlo.p, dhi.p, dtr.p, max_trace, dres.p, st));
+    int res[2];
+    NH_HIP(hipMemcpy(res, dres.p, sizeof(res), hipMemcpyDeviceToHost));
+    int rows = res[1] < max_trace ? res[1] : max_trace;
+    if (rows > 0)
+      NH_HIP(hipMemcpy(trace, dtr.p, sizeof(double) * (size_t)rows * (n + 1),
+                       hipMemcpyDeviceToHost));
+    *evals_out = res[1];
+    return res[0];
+  } catch (const std::exception& e) {
+    put_err(err, err_len, e.what());
+    return -100 - NH_ERR_GPU;
+  }
+}
+
+}  // extern "C"

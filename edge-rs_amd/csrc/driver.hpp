@@ -1,0 +1,55 @@
+// Per-run host state shared by the driver's stages.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/nebula_hip.h"
+#include "host_common.hpp"
+
+namespace nh {
+
+struct Resident;
+
+struct GeneInfo {
+  double mean_count = 0.0;     // nebula's mct
+  std::vector<double> totals;  // per-subject count totals
+  std::vector<double> xty;     // sum over positive cells of x * y (centred design)
+  CountTail tail;              // lgamma tail histogram and n1 / n2
+};
+
+struct GenePlan {
+  unsigned ord = 1;
+  double sigma = 0, gamma = 0;  // stage one's variance components
+  int convergence = 1;
+  std::vector<double> beta_start;
+  int refit = 1;  // 0 none, 1 both, 2 subject only
+  bool searched = false, found = false, failed = false;
+  double v[2] = {0, 0};  // stage-two minimiser
+  int evaluations = 0;
+  double objective = 0;
+};
+
+struct Prepared {
+  size_t n_genes = 0, n_cells = 0, n_coef = 0, k = 0, intercept = 0;
+  nebula_params_t params;
+  std::vector<size_t> fid;
+  std::vector<double> log_offset, centred, sds;
+  double log_mean_offset = 0, cv2 = 0, cells_per_subject = 0;
+  int method = NH_METHOD_HL;
+  bool tables = false;
+  std::vector<size_t> cell_columns, kept;
+  // kept genes' positive counts, CSR over kept genes
+  std::vector<uint64_t> gene_ptr;
+  std::vector<uint32_t> cells;
+  std::vector<double> counts;
+  std::vector<GeneInfo> genes;
+  std::vector<GenePlan> plans;
+};
+
+double digamma(double x);
+double trigamma(double x);
+void run_stage_two_batched(Prepared& P, Resident& R, const nebula_hip_options_t& o,
+                           nebula_run_stats_t& st);
+
+}  // namespace nh

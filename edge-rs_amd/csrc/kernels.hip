@@ -1,0 +1,162 @@
+// gfx950 kernels of the stage-two path and their host-side launchers.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+
+#include "bobyqa.hpp"
+#include "kernels.hpp"
+#include "pml_fit.hpp"
+
+namespace nh {
+
+// ---------------------------------------------------------------------------
+// One workgroup per request: a penalised fit at fixed (subject, cell)
+// (edge-rs ResidentBatch::solve, src/gpu/nebula_gpu.rs:441, one fit per plane
+// there; one per workgroup here).
+template <int NB, int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void pml_batch_kernel(
+    DeviceData dd, const DevRequest* req, const double* beta_init, const double* lw_init,
+    DevReply* out, double* out_beta, double* out_lw, double* out_info, double* scratch,
+    size_t stride, FitKnobs kn, int mode, FitCounters* ctr) {
+  __shared__ FitLds<NB, WAVES> sh;
+  const int q = blockIdx.x;
+  const int k = (int)dd.k;
+  FitScratch<NB> sc;
+  sc.carve(scratch + (size_t)q * stride, k, WAVES);
+  const DevRequest r = req[q];
+  if (threadIdx.x < NB) sh.beta[threadIdx.x] = beta_init[(size_t)q * NB + threadIdx.x];
+  for (int s = threadIdx.x; s < k; s += WAVES * 64)
+    sc.lw[s] = lw_init ? lw_init[(size_t)q * k + s] : 0.0;
+  __syncthreads();
+  const double exps = exp(r.subject);
+  const double alpha = 1.0 / (exps - 1.0);
+  const double lambda = 1.0 / (sqrt(exps) * (exps - 1.0));
+  Fitter<NB, WAVES> f(dd, sh, sc, ctr, r.gene, r.cell, alpha, lambda);
+  FitResult res = f.run(mode, r.ord, kn);
+  if (threadIdx.x == 0) {
+    DevReply o;
+    o.ll = res.ll;
+    o.ll_prev = res.ll_prev;
+    o.log_det = res.log_det;
+    o.second = res.second;
+    o.iterations = res.iterations;
+    o.backtracks = res.backtracks;
+    o.finish_steps = res.finish_steps;
+    o.exact = res.exact;
+    out[q] = o;
+  }
+  if (threadIdx.x < NB) out_beta[(size_t)q * NB + threadIdx.x] = sh.beta[threadIdx.x];
+  if (threadIdx.x < NB * NB) out_info[(size_t)q * NB * NB + threadIdx.x] = sh.vb2[threadIdx.x];
+  for (int s = threadIdx.x; s < k; s += WAVES * 64) out_lw[(size_t)q * k + s] = sc.lw[s];
+}
+
+size_t fit_scratch_doubles(int nb, int k, int waves) {
+  switch (nb) {
+    case 1: return FitScratch<1>::doubles(k, waves);
+    case 2: return FitScratch<2>::doubles(k, waves);
+    case 3: return FitScratch<3>::doubles(k, waves);
+    case 4: return FitScratch<4>::doubles(k, waves);
+    case 5: return FitScratch<5>::doubles(k, waves);
+    case 6: return FitScratch<6>::doubles(k, waves);
+    case 7: return FitScratch<7>::doubles(k, waves);
+    default: return FitScratch<8>::doubles(k, waves);
+  }
+}
+
+template <int NB, int WAVES>
+static hipError_t launch_batch_t(const BatchLaunch& L, hipStream_t st) {
+  hipLaunchKernelGGL((pml_batch_kernel<NB, WAVES>), dim3(L.n_req), dim3(WAVES * 64), 0, st, L.dd,
+                     L.req, L.beta_init, L.lw_init, L.out, L.out_beta, L.out_lw, L.out_info,
+                     L.scratch, L.stride, L.knobs, L.mode, L.counters);
+  return hipGetLastError();
+}
+
+template <int NB>
+static hipError_t launch_batch_nb(const BatchLaunch& L, hipStream_t st) {
+  switch (L.waves) {
+    case 4: return launch_batch_t<NB, 4>(L, st);
+    case 16: return launch_batch_t<NB, 16>(L, st);
+    default: return launch_batch_t<NB, 8>(L, st);
+  }
+}
+
+hipError_t launch_batch(const BatchLaunch& L, hipStream_t st) {
+  if (L.n_req == 0) return hipSuccess;
+  switch (L.dd.nb) {
+    case 1: return launch_batch_nb<1>(L, st);
+    case 2: return launch_batch_nb<2>(L, st);
+    case 3: return launch_batch_nb<3>(L, st);
+    case 4: return launch_batch_nb<4>(L, st);
+    case 5: return launch_batch_nb<5>(L, st);
+    case 6: return launch_batch_nb<6>(L, st);
+    case 7: return launch_batch_nb<7>(L, st);
+    case 8: return launch_batch_nb<8>(L, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BOBYQA driven entirely on the device against built-in test objectives, so
+// the device arithmetic can be checked against the nloptr traces.
+__device__ static double test_objective(int problem, const double* x) {
+  NH_STRICT_FP
+  switch (problem) {
+    case 0: {
+      double a = x[1] - x[0] * x[0], b = 1.0 - x[0];
+      return 100.0 * (a * a) + b * b;
+    }
+    case 1: {
+      double a = x[0] - 3.0, b = x[1] + 1.0;
+      return a * a + 10.0 * (b * b) + x[0] * x[1];
+    }
+    case 2: {
+      double u = 10000.0 * x[0] + 30000.0 * x[1];
+      double t = fabs(u - 2.0 * floor((u + 1.0) / 2.0));
+      double a = x[0] - 0.25, b = x[1] + 0.125;
+      return a * a + 2.0 * (b * b) + 0.5 * x[0] * x[1] + 0.5 * (t * t);
+    }
+    case 3: {
+      double a = x[0] + x[1] - 1.0;
+      return a * a;
+    }
+    default: {
+      double a = x[0] - 1.0, b = x[1] - x[0] * x[0], c = x[2] - x[1], d = x[2] * x[2];
+      return a * a + 100.0 * (b * b) + c * c + 0.5 * (d * d);
+    }
+  }
+}
+
+__global__ void bobyqa_trace_kernel(int problem, int n, const double* x0, const double* lo,
+                                    const double* hi, double* trace, int max_trace,
+                                    int* result) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  Bobyqa<3> b;
+  b.init(n, x0, lo, hi);
+  int i = 0;
+  while (b.want) {
+    double v;
+    {
+      NH_STRICT_FP
+      v = test_objective(problem, b.xask);
+    }
+    if (i < max_trace) {
+      for (int j = 0; j < n; ++j) trace[(size_t)i * (n + 1) + j] = b.xask[j];
+      trace[(size_t)i * (n + 1) + n] = v;
+    }
+    ++i;
+    b.tell(v);
+  }
+  result[0] = b.status;
+  result[1] = i;
+}
+
+hipError_t launch_bobyqa_trace(int problem, int n, const double* x0, const double* lo,
+                               const double* hi, double* trace, int max_trace, int* result,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(bobyqa_trace_kernel, dim3(1), dim3(64), 0, st, problem, n, x0, lo, hi,
+                     trace, max_trace, result);
+  return hipGetLastError();
+}
+
+}  // namespace nh

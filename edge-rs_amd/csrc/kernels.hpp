@@ -1,0 +1,48 @@
+// Host-visible declarations of the device data layout and launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "pml_fit.hpp"
+
+namespace nh {
+
+struct DevRequest {
+  uint32_t gene;
+  uint32_t ord;
+  double subject;  // nebula's sigma[0]
+  double cell;     // NB size gamma
+};
+
+struct DevReply {
+  double ll, ll_prev, log_det, second;
+  int iterations, backtracks, finish_steps, exact;
+};
+
+struct BatchLaunch {
+  DeviceData dd;
+  const DevRequest* req;
+  const double* beta_init;  // n_req * nb
+  const double* lw_init;    // n_req * k or null
+  DevReply* out;
+  double* out_beta;
+  double* out_lw;
+  double* out_info;
+  double* scratch;
+  size_t stride;  // doubles per request
+  FitKnobs knobs;
+  int mode;       // 0 f32 + f64 finish, 1 f64
+  int waves;      // 4, 8 or 16
+  int n_req;
+  FitCounters* counters;
+};
+
+size_t fit_scratch_doubles(int nb, int k, int waves);
+hipError_t launch_batch(const BatchLaunch& L, hipStream_t st);
+hipError_t launch_bobyqa_trace(int problem, int n, const double* x0, const double* lo,
+                               const double* hi, double* trace, int max_trace, int* result,
+                               hipStream_t st);
+
+}  // namespace nh

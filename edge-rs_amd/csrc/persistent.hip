@@ -1,0 +1,342 @@
+// Stage two, batched engine: a persistent kernel whose workgroups pull genes
+// off a device work queue (longest first) and run each gene's whole search on
+// the device, BOBYQA included.
+//
+// The reference runs its stage-two searches in lockstep host rounds
+// (src/gpu/stage_two.rs:362-544): every round waits for the slowest fit and
+// for the host's f64 finish, so the slowest gene sets the round count and the
+// host sets the pace.  Here lane 0 of a workgroup steps the gene's BOBYQA
+// (the same stepper as the host's, edge-rs-amd/csrc/bobyqa.hpp, contraction
+// off so the device arithmetic is NLopt's), the whole workgroup runs the
+// penalised fit at each asked point (FP32 Newton + FP64 finish), assembles
+// the profile objective in FP64 (VarianceObjective::assemble,
+// src/sc/nebula.rs:1452-1477, lgamma tail over the gene's count histogram)
+// and tells the stepper; a finished workgroup takes the next gene.  No host
+// round trip until every gene is done.
+//
+// Search semantics are the host plain engine's: the order-one retry and the
+// abandon-on-non-finite rule of StageTwoSearch (nebula.rs:1656-1741), inner
+// fits warm-started from the gene's first finite fit (stage_two.rs:36-43).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "bobyqa.hpp"
+#include "driver.hpp"
+#include "pml_fit.hpp"
+#include "persistent.hpp"
+#include "resident.hpp"
+
+namespace nh {
+
+// Profile objective on the device, same arithmetic as host_common.hpp's.
+__device__ static double device_objective(double subject, double cell, const FitResult& r,
+                                          double n_cells, double k, double tail, double n_pos,
+                                          double n_one, double n_two) {
+  if (r.second < -1.0) return INFINITY;
+  const double exps = exp(subject);
+  const double alpha = 1.0 / (exps - 1.0);
+  const double lambda = 1.0 / (sqrt(exps) * (exps - 1.0));
+  const double base = isnan(r.ll) ? r.ll_prev : r.ll;
+  double v = base + n_cells * cell * log(cell) + k * alpha * log(lambda) - k * lgamma(alpha);
+  v += tail - (n_pos - n_one - n_two) * lgamma(cell) + (n_one + n_two) * log(cell) +
+       n_two * log(cell + 1.0);
+  v += -0.5 * r.log_det + log(1.0 + r.second);
+  return isfinite(v) ? -v : INFINITY;
+}
+
+template <int NB, int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void stage_two_kernel(PersistentArgs A) {
+  constexpr int NT = WAVES * 64;
+  __shared__ FitLds<NB, WAVES> sh;
+  __shared__ Bobyqa<2> bq;
+  __shared__ double warm_beta[NB];
+  __shared__ double red[NT / 64];
+  __shared__ int s_slot, s_active, s_found, s_failed, s_have_warm, s_evals, s_order, s_ok;
+  __shared__ double s_subject, s_cell, s_value;
+  const int tid = threadIdx.x;
+  const DeviceData& dd = A.dd;
+  const int k = (int)dd.k;
+  FitScratch<NB> sc;
+  double* base = A.scratch + (size_t)blockIdx.x * A.stride;
+  sc.carve(base, k, WAVES);
+  double* warm_lw = base + FitScratch<NB>::doubles(k, WAVES);
+  for (;;) {
+    if (tid == 0) s_slot = (int)atomicAdd(A.queue, 1u);
+    __syncthreads();
+    const int slot = s_slot;
+    __syncthreads();
+    if (slot >= A.n_search) break;
+    const DevSearch S = A.search[A.order[slot]];
+    if (tid == 0) {
+      s_order = (int)S.ord;
+      s_have_warm = 0;
+      s_evals = 0;
+      s_found = 0;
+      s_failed = 0;
+      const double lo[2] = {A.lo[0], A.lo[1]}, hi[2] = {A.hi[0], A.hi[1]};
+      double x0[2] = {fmin(fmax(S.start[0], lo[0]), hi[0]), fmin(fmax(S.start[1], lo[1]), hi[1])};
+      bq.init(2, x0, lo, hi);
+      s_active = bq.status != BOBYQA_BAD_BOX && bq.want;
+    }
+    __syncthreads();
+    while (s_active) {
+      // request for the asked point
+      if (tid == 0) {
+        s_subject = bq.xask[0];
+        s_cell = bq.xask[1];
+      }
+      __syncthreads();
+      const double subject = s_subject, cell = s_cell;
+      if (tid < NB) {
+        double b = S.beta_start[tid];
+        if (tid == (int)A.intercept) b -= subject / 2.0;
+        sh.beta[tid] = s_have_warm ? warm_beta[tid] : b;
+      }
+      for (int s = tid; s < k; s += NT) sc.lw[s] = s_have_warm ? warm_lw[s] : 0.0;
+      __syncthreads();
+      const double exps = exp(subject);
+      const double alpha = 1.0 / (exps - 1.0);
+      const double lambda = 1.0 / (sqrt(exps) * (exps - 1.0));
+      Fitter<NB, WAVES> f(dd, sh, sc, A.counters, S.gene, cell, alpha, lambda);
+      const FitResult r = f.run(0, (unsigned)s_order, A.knobs);
+      // finite point and the lgamma tail over the count histogram
+      double part = 0.0;
+      bool fin = true;
+      for (int s = tid; s < k; s += NT) fin = fin && isfinite(sc.lw[s]);
+      if (tid < NB) fin = fin && isfinite(sh.beta[tid]);
+      for (uint32_t i = (uint32_t)tid; i < S.tail_len; i += NT)
+        part += A.tail_mult[S.tail_off + i] * lgamma(A.tail_val[S.tail_off + i] + cell);
+      part = wave_sum(part);
+      const bool wfin = __all(fin);
+      if ((tid % 64) == 0) red[tid / 64] = wfin ? part : NAN;
+      __syncthreads();
+      if (tid == 0) {
+        double tail = 0.0;
+        bool allfin = true;
+        for (int w = 0; w < WAVES; ++w) {
+          if (isnan(red[w])) allfin = false;
+          else tail += red[w];
+        }
+        double v = INFINITY;
+        if (allfin)
+          v = device_objective(subject, cell, r, (double)dd.n_cells, (double)k, tail, S.n_pos,
+                               S.n_one, S.n_two);
+        s_value = v;
+        s_evals += 1;
+        s_ok = isfinite(v) && !s_have_warm;
+      }
+      __syncthreads();
+      if (s_ok) {
+        if (tid < NB) warm_beta[tid] = sh.beta[tid];
+        for (int s = tid; s < k; s += NT) warm_lw[s] = sc.lw[s];
+      }
+      __syncthreads();
+      if (tid == 0) {
+        if (s_ok) s_have_warm = 1;
+        const double v = s_value;
+        bool pass_over = false, found = false;
+        if (!isfinite(v)) {
+          pass_over = true;
+        } else {
+          bq.tell(v);
+          if (!bq.want) {
+            pass_over = true;
+            found = true;
+          }
+        }
+        if (pass_over) {
+          if (found || s_order == 1) {
+            s_active = 0;
+            s_found = found ? 1 : 0;
+            s_failed = bq.status == BOBYQA_ROUNDOFF_LIMITED ? 1 : 0;
+          } else {
+            s_order = 1;
+            const double lo[2] = {A.lo[0], A.lo[1]}, hi[2] = {A.hi[0], A.hi[1]};
+            double x0[2] = {fmin(fmax(S.start[0], lo[0]), hi[0]),
+                            fmin(fmax(S.start[1], lo[1]), hi[1])};
+            bq.init(2, x0, lo, hi);
+            s_active = bq.status != BOBYQA_BAD_BOX && bq.want;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      DevSearchOut o;
+      o.x[0] = bq.xbest_out[0];
+      o.x[1] = bq.xbest_out[1];
+      o.f = bq.fbest_out;
+      o.found = s_found;
+      o.failed = s_failed;
+      o.evals = s_evals;
+      o.ord = s_order;
+      A.out[A.order[slot]] = o;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void reset_queue_kernel(unsigned* q) {
+  if (threadIdx.x == 0) *q = 0u;
+}
+
+size_t persistent_stride(int nb, int k, int waves) { return fit_scratch_doubles(nb, k, waves) + (size_t)k; }
+
+template <int NB, int WAVES>
+static hipError_t launch_t(const PersistentArgs& A, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, st, A.queue);
+  hipLaunchKernelGGL((stage_two_kernel<NB, WAVES>), dim3(grid), dim3(WAVES * 64), 0, st, A);
+  return hipGetLastError();
+}
+
+template <int NB>
+static hipError_t launch_nb(const PersistentArgs& A, int waves, int grid, hipStream_t st) {
+  switch (waves) {
+    case 4: return launch_t<NB, 4>(A, grid, st);
+    case 16: return launch_t<NB, 16>(A, grid, st);
+    default: return launch_t<NB, 8>(A, grid, st);
+  }
+}
+
+hipError_t launch_persistent(const PersistentArgs& A, int waves, int grid, hipStream_t st) {
+  switch (A.dd.nb) {
+    case 1: return launch_nb<1>(A, waves, grid, st);
+    case 2: return launch_nb<2>(A, waves, grid, st);
+    case 3: return launch_nb<3>(A, waves, grid, st);
+    case 4: return launch_nb<4>(A, waves, grid, st);
+    case 5: return launch_nb<5>(A, waves, grid, st);
+    case 6: return launch_nb<6>(A, waves, grid, st);
+    case 7: return launch_nb<7>(A, waves, grid, st);
+    case 8: return launch_nb<8>(A, waves, grid, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side: build the search descriptors, order genes longest first, launch
+// once (the launch pair is captured in a hipGraph and replayed), read back.
+void run_stage_two_batched(Prepared& P, Resident& R, const nebula_hip_options_t& o,
+                           nebula_run_stats_t& st) {
+  const size_t G = P.kept.size(), nb = P.n_coef, k = P.k;
+  const nebula_params_t& p = P.params;
+  std::vector<DevSearch> hs;
+  std::vector<uint32_t> idx_of;  // search -> kept index
+  std::vector<double> tv, tm;
+  for (size_t i = 0; i < G; ++i) {
+    if (P.plans[i].refit != 1) continue;
+    const GenePlan& pl = P.plans[i];
+    const GeneInfo& gi = P.genes[i];
+    DevSearch d{};
+    d.gene = (uint32_t)i;
+    d.ord = pl.ord;
+    d.start[0] = pl.sigma;
+    d.start[1] = pl.gamma;
+    for (size_t j = 0; j < nb; ++j) d.beta_start[j] = pl.beta_start[j];
+    d.tail_off = tv.size();
+    d.tail_len = (uint32_t)gi.tail.value.size();
+    tv.insert(tv.end(), gi.tail.value.begin(), gi.tail.value.end());
+    tm.insert(tm.end(), gi.tail.mult.begin(), gi.tail.mult.end());
+    d.n_pos = gi.tail.n_pos;
+    d.n_one = gi.tail.n_one;
+    d.n_two = gi.tail.n_two;
+    hs.push_back(d);
+    idx_of.push_back((uint32_t)i);
+    P.plans[i].searched = true;
+  }
+  const size_t n = hs.size();
+  st.genes_searched = n;
+  if (n == 0) return;
+  // longest first: work per evaluation ~ cells + positive counts
+  std::vector<uint32_t> order(n);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    const size_t na = P.gene_ptr[hs[a].gene + 1] - P.gene_ptr[hs[a].gene];
+    const size_t nb2 = P.gene_ptr[hs[b].gene + 1] - P.gene_ptr[hs[b].gene];
+    return na > nb2;
+  });
+  int dev_cus = 0;
+  NH_HIP(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, R.device));
+  // tile shape: one gene per workgroup; wide workgroups when genes are few
+  int waves = 8;
+  if (o.wg_size >= 1024)
+    waves = 16;
+  else if (o.wg_size > 0 && o.wg_size <= 256)
+    waves = 4;
+  else if (o.wg_size == 0 && (int)n <= dev_cus)
+    waves = 16;
+  const int per_cu = waves == 16 ? 1 : (waves == 8 ? 2 : 4);
+  const int grid = (int)std::min<size_t>(n, (size_t)std::max(1, dev_cus * per_cu));
+  DevBuf<DevSearch> dsearch;
+  DevBuf<uint32_t> dorder;
+  DevBuf<unsigned> dqueue;
+  DevBuf<double> dtv, dtm, dscr;
+  DevBuf<DevSearchOut> dout;
+  dsearch.upload(hs.data(), n, R.stream);
+  dorder.upload(order.data(), n, R.stream);
+  dtv.upload(tv.data(), tv.size(), R.stream);
+  dtm.upload(tm.data(), tm.size(), R.stream);
+  dqueue.alloc(1);
+  dout.alloc(n);
+  const size_t stride = persistent_stride((int)nb, (int)k, waves);
+  dscr.alloc(stride * (size_t)grid);
+  PersistentArgs A;
+  A.dd = R.view();
+  A.search = dsearch.p;
+  A.order = dorder.p;
+  A.n_search = (int)n;
+  A.queue = dqueue.p;
+  A.tail_val = dtv.p;
+  A.tail_mult = dtm.p;
+  A.out = dout.p;
+  A.scratch = dscr.p;
+  A.stride = stride;
+  A.knobs.eps = p.eps;
+  A.knobs.noise_rel = 1e-8;
+  A.knobs.max_iter = 50;
+  A.knobs.max_backtrack = 10;
+  A.lo[0] = p.min_subject;
+  A.lo[1] = p.min_cell;
+  A.hi[0] = p.max_subject;
+  A.hi[1] = p.max_cell;
+  A.intercept = (uint32_t)P.intercept;
+  A.counters = R.counters.p;
+  // capture the queue reset + persistent launch once, replay it
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  NH_HIP(hipStreamBeginCapture(R.stream, hipStreamCaptureModeThreadLocal));
+  hipError_t le = launch_persistent(A, waves, grid, R.stream);
+  hipError_t ce = hipStreamEndCapture(R.stream, &graph);
+  if (le != hipSuccess) throw HipError(std::string("launch_persistent: ") + hipGetErrorString(le));
+  NH_HIP(ce);
+  NH_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  NH_HIP(hipEventRecord(R.ev0, R.stream));
+  NH_HIP(hipGraphLaunch(exec, R.stream));
+  NH_HIP(hipEventRecord(R.ev1, R.stream));
+  std::vector<DevSearchOut> ho(n);
+  NH_HIP(hipMemcpyAsync(ho.data(), dout.p, n * sizeof(DevSearchOut), hipMemcpyDeviceToHost,
+                        R.stream));
+  NH_HIP(hipStreamSynchronize(R.stream));
+  float ms = 0.0f;
+  NH_HIP(hipEventElapsedTime(&ms, R.ev0, R.ev1));
+  R.kernel_ms += ms;
+  R.launches += 1;
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  st.rounds = 1;
+  for (size_t q = 0; q < n; ++q) {
+    GenePlan& pl = P.plans[idx_of[q]];
+    pl.found = ho[q].found != 0;
+    pl.failed = ho[q].failed != 0;
+    pl.v[0] = ho[q].x[0];
+    pl.v[1] = ho[q].x[1];
+    pl.objective = ho[q].f;
+    pl.evaluations = ho[q].evals;
+    st.evaluations += (uint64_t)ho[q].evals;
+  }
+}
+
+}  // namespace nh

@@ -1,0 +1,47 @@
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pml_fit.hpp"
+
+namespace nh {
+
+// One gene's stage-two search, as the persistent kernel reads it.
+struct DevSearch {
+  uint32_t gene;
+  uint32_t ord;          // Laplace order of the first pass
+  double start[2];       // stage one's (sigma, gamma)
+  double beta_start[8];  // fixed effects every cold fit starts from
+  uint64_t tail_off;     // count histogram slice (values != 1, 2)
+  uint32_t tail_len;
+  double n_pos, n_one, n_two;
+};
+
+struct DevSearchOut {
+  double x[2];
+  double f;
+  int found, failed, evals, ord;
+};
+
+struct PersistentArgs {
+  DeviceData dd;
+  const DevSearch* search;
+  const uint32_t* order;  // work order, longest gene first
+  int n_search;
+  unsigned* queue;        // next slot
+  const double* tail_val;
+  const double* tail_mult;
+  DevSearchOut* out;
+  double* scratch;
+  size_t stride;          // doubles per workgroup
+  FitKnobs knobs;
+  double lo[2], hi[2];
+  uint32_t intercept;
+  FitCounters* counters;
+};
+
+size_t persistent_stride(int nb, int k, int waves);
+hipError_t launch_persistent(const PersistentArgs& A, int waves, int grid, hipStream_t st);
+
+}  // namespace nh

@@ -1,0 +1,158 @@
+// Host side of the resident batch: upload once, solve many request batches.
+#include "resident.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace nh {
+
+Resident::~Resident() {
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Resident::upload(size_t nc, size_t nb_, size_t k_, const double* design,
+                      const double* log_offset, const uint64_t* subject_start, size_t ng,
+                      const uint64_t* gene_ptr, const uint32_t* cells, const double* counts,
+                      int dev) {
+  if (nb_ < 1 || nb_ > 8) throw std::invalid_argument("the device fit takes 1 to 8 design columns");
+  if (nc >= 0xffffffffull) throw std::invalid_argument("too many cells for u32 cell indices");
+  device = dev;
+  NH_HIP(hipSetDevice(device));
+  if (!stream) NH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  if (!ev0) NH_HIP(hipEventCreate(&ev0));
+  if (!ev1) NH_HIP(hipEventCreate(&ev1));
+  n_cells = nc;
+  nb = nb_;
+  p4 = (nb + 3) / 4 * 4;
+  k = k_;
+  n_genes = ng;
+  std::vector<float> h32(n_cells * p4, 0.0f), ho32(n_cells);
+  for (size_t i = 0; i < n_cells; ++i) {
+    for (size_t j = 0; j < nb; ++j) h32[i * p4 + j] = (float)design[i * nb + j];
+    ho32[i] = (float)log_offset[i];
+  }
+  x32.upload(h32.data(), h32.size(), stream);
+  off32.upload(ho32.data(), ho32.size(), stream);
+  x64.upload(design, n_cells * nb, stream);
+  off64.upload(log_offset, n_cells, stream);
+  std::vector<uint32_t> hf(k + 1);
+  for (size_t s = 0; s <= k; ++s) hf[s] = (uint32_t)subject_start[s];
+  fid.upload(hf.data(), hf.size(), stream);
+  const size_t nnz = gene_ptr[n_genes];
+  gptr.upload(gene_ptr, n_genes + 1, stream);
+  gcell.upload(cells, nnz, stream);
+  gy64.upload(counts, nnz, stream);
+  std::vector<float> hy(nnz);
+  for (size_t i = 0; i < nnz; ++i) hy[i] = (float)counts[i];
+  gy32.upload(hy.data(), nnz, stream);
+  // subject totals per gene
+  std::vector<double> tot(n_genes * k, 0.0);
+  for (size_t g = 0; g < n_genes; ++g) {
+    size_t s = 0;
+    for (uint64_t i = gene_ptr[g]; i < gene_ptr[g + 1]; ++i) {
+      while (cells[i] >= hf[s + 1]) ++s;
+      tot[g * k + s] += counts[i];
+    }
+  }
+  gtot.upload(tot.data(), tot.size(), stream);
+  counters.alloc(1);
+  reset_counters();
+  NH_HIP(hipStreamSynchronize(stream));
+}
+
+DeviceData Resident::view() const {
+  DeviceData d;
+  d.x32 = x32.p;
+  d.x64 = x64.p;
+  d.off32 = off32.p;
+  d.off64 = off64.p;
+  d.fid = fid.p;
+  d.gptr = gptr.p;
+  d.gcell = gcell.p;
+  d.gy32 = gy32.p;
+  d.gy64 = gy64.p;
+  d.gtot = gtot.p;
+  d.n_cells = (uint32_t)n_cells;
+  d.nb = (uint32_t)nb;
+  d.p4 = (uint32_t)p4;
+  d.k = (uint32_t)k;
+  return d;
+}
+
+void Resident::reset_counters() {
+  FitCounters z;
+  std::memset(&z, 0, sizeof(z));
+  NH_HIP(hipMemcpyAsync(counters.p, &z, sizeof(z), hipMemcpyHostToDevice, stream));
+  NH_HIP(hipStreamSynchronize(stream));
+}
+
+FitCounters Resident::read_counters() {
+  FitCounters c;
+  NH_HIP(hipMemcpyAsync(&c, counters.p, sizeof(c), hipMemcpyDeviceToHost, stream));
+  NH_HIP(hipStreamSynchronize(stream));
+  return c;
+}
+
+void Resident::solve(const std::vector<DevRequest>& r, const std::vector<double>& bi,
+                     const std::vector<double>* lwi, double eps, int mode, int waves,
+                     std::vector<DevReply>& out, std::vector<double>& ob,
+                     std::vector<double>& olw, std::vector<double>* oinfo) {
+  const size_t n = r.size();
+  out.resize(n);
+  ob.resize(n * nb);
+  olw.resize(n * k);
+  if (oinfo) oinfo->resize(n * nb * nb);
+  if (n == 0) return;
+  for (const DevRequest& q : r)
+    if (q.gene >= n_genes) throw std::invalid_argument("request names a gene that is not resident");
+  NH_HIP(hipSetDevice(device));
+  req.upload(r.data(), n, stream);
+  beta_init.upload(bi.data(), n * nb, stream);
+  if (lwi) lw_init.upload(lwi->data(), n * k, stream);
+  reply.alloc(n);
+  out_beta.alloc(n * nb);
+  out_lw.alloc(n * k);
+  out_info.alloc(n * nb * nb);
+  const size_t stride = fit_scratch_doubles((int)nb, (int)k, waves);
+  scratch.alloc(stride * n);
+  BatchLaunch L;
+  L.dd = view();
+  L.req = req.p;
+  L.beta_init = beta_init.p;
+  L.lw_init = lwi ? lw_init.p : nullptr;
+  L.out = reply.p;
+  L.out_beta = out_beta.p;
+  L.out_lw = out_lw.p;
+  L.out_info = out_info.p;
+  L.scratch = scratch.p;
+  L.stride = stride;
+  L.knobs.eps = eps;
+  L.knobs.noise_rel = 1e-8;
+  L.knobs.max_iter = 50;
+  L.knobs.max_backtrack = 10;
+  L.mode = mode;
+  L.waves = waves;
+  L.n_req = (int)n;
+  L.counters = counters.p;
+  NH_HIP(hipEventRecord(ev0, stream));
+  NH_HIP(launch_batch(L, stream));
+  NH_HIP(hipEventRecord(ev1, stream));
+  NH_HIP(hipMemcpyAsync(out.data(), reply.p, n * sizeof(DevReply), hipMemcpyDeviceToHost, stream));
+  NH_HIP(hipMemcpyAsync(ob.data(), out_beta.p, n * nb * sizeof(double), hipMemcpyDeviceToHost,
+                        stream));
+  NH_HIP(hipMemcpyAsync(olw.data(), out_lw.p, n * k * sizeof(double), hipMemcpyDeviceToHost,
+                        stream));
+  if (oinfo)
+    NH_HIP(hipMemcpyAsync(oinfo->data(), out_info.p, n * nb * nb * sizeof(double),
+                          hipMemcpyDeviceToHost, stream));
+  NH_HIP(hipStreamSynchronize(stream));
+  float ms = 0.0f;
+  NH_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+  kernel_ms += ms;
+  ++launches;
+}
+
+}  // namespace nh

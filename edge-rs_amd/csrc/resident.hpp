@@ -1,0 +1,83 @@
+// Device-resident copy of one run's shared inputs and per-gene counts, and
+// batched penalised fits on it (edge-rs ResidentBatch, src/gpu/nebula_gpu.rs:195-648).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace nh {
+
+struct HipError : std::runtime_error {
+  explicit HipError(const std::string& m) : std::runtime_error(m) {}
+};
+
+#define NH_HIP(call)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      throw ::nh::HipError(std::string(#call) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+// A device allocation that frees itself.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    release();
+    NH_HIP(hipMalloc((void**)&p, sizeof(T) * (count ? count : 1)));
+    n = count;
+  }
+  void upload(const T* h, size_t count, hipStream_t st) {
+    alloc(count);
+    if (count) NH_HIP(hipMemcpyAsync(p, h, sizeof(T) * count, hipMemcpyHostToDevice, st));
+  }
+};
+
+struct Resident {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t n_cells = 0, nb = 0, p4 = 0, k = 0, n_genes = 0;
+  DevBuf<float> x32, off32, gy32;
+  DevBuf<double> x64, off64, gy64, gtot;
+  DevBuf<uint32_t> fid, gcell;
+  DevBuf<uint64_t> gptr;
+  DevBuf<FitCounters> counters;
+  // per-solve staging
+  DevBuf<DevRequest> req;
+  DevBuf<DevReply> reply;
+  DevBuf<double> beta_init, lw_init, out_beta, out_lw, out_info, scratch;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double kernel_ms = 0.0;
+  unsigned long long launches = 0;
+
+  ~Resident();
+  void upload(size_t n_cells, size_t nb, size_t k, const double* design, const double* log_offset,
+              const uint64_t* subject_start, size_t n_genes, const uint64_t* gene_ptr,
+              const uint32_t* cells, const double* counts, int device);
+  DeviceData view() const;
+  // Runs n requests, replies filled on the host.
+  void solve(const std::vector<DevRequest>& r, const std::vector<double>& beta_init,
+             const std::vector<double>* lw_init, double eps, int mode, int waves,
+             std::vector<DevReply>& out, std::vector<double>& out_beta,
+             std::vector<double>& out_lw, std::vector<double>* out_info);
+  FitCounters read_counters();
+  void reset_counters();
+};
+
+}  // namespace nh

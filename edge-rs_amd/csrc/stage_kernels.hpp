@@ -1,0 +1,35 @@
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pml_fit.hpp"
+
+namespace nh {
+
+struct PtmgRequest {
+  uint32_t gene;
+  uint32_t hess;  // 1: also the Hessian sums
+};
+
+struct PtmgLaunch {
+  DeviceData dd;
+  const PtmgRequest* req;
+  const double* params;  // n_req * (nb + 2): [beta, sigma, phi]
+  double* out;           // n_req * ptmg_out_doubles(nb, k)
+  double* scratch;
+  size_t stride;
+  int n_req;
+  FitCounters* counters;
+};
+
+// Layout of one request's output: per subject s (PS = 4 + 4 nb doubles at
+// s * PS): sum e, sum g e, sum (g e/w - e/w), sum g e^2/w, sum x e,
+// sum g x e, sum x e (g/w - 1/w), sum x e g e/w; then per gene (at k * PS):
+// sum y eta, sum (ym e - (gamma + y) ln w), sum ln w, sum g, sum 1/w,
+// sum g/w, Gram[nb (nb + 1) / 2] (row-major upper triangle).
+size_t ptmg_out_doubles(int nb, int k);
+size_t ptmg_scratch_doubles(int nb, int k, int waves);
+hipError_t launch_ptmg(const PtmgLaunch& L, hipStream_t st);
+
+}  // namespace nh

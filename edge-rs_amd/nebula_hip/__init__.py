@@ -1,0 +1,521 @@
+"""Python mirror of edge-rs's NEBULA stage-two interface on the HIP library.
+
+    from nebula_hip import nebula_sparse_hip, NebulaParams, CompressedSparse
+
+Same names, argument meaning and errors as edge-rs's ``nebula_sparse`` /
+``nebula_sparse_gpu`` (src/sc/nebula.rs:434-445, src/gpu/stage_two.rs:233-265)
+and ``ResidentBatch`` (src/gpu/nebula_gpu.rs:195-648).  Every call goes through
+the C ABI of libnebula_hip.so (include/nebula_hip.h); there is no host
+fallback: if the library is missing, loading raises.
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnebula_hip.so")
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_u64p = C.POINTER(C.c_uint64)
+_u32p = C.POINTER(C.c_uint32)
+
+ENGINE_PLAIN = 0
+ENGINE_BATCHED = 1
+
+
+# ---------------------------------------------------------------------------
+# Errors: one class per EdgeErrors variant on this path (src/errors.rs:13-223).
+class EdgeError(RuntimeError):
+    code = -1
+
+    def __init__(self, message=""):
+        super().__init__(message)
+
+
+def _err(name, code):
+    return type(name, (EdgeError,), {"code": code})
+
+
+EmptyCounts = _err("EmptyCounts", 1)
+InvalidArgument = _err("InvalidArgument", 2)
+MustBePositive = _err("MustBePositive", 3)
+LengthMismatch = _err("LengthMismatch", 4)
+ShapeMismatch = _err("ShapeMismatch", 5)
+MalformedSparse = _err("MalformedSparse", 6)
+SubjectsNotContiguous = _err("SubjectsNotContiguous", 7)
+TooFewSubjects = _err("TooFewSubjects", 8)
+MissingIntercept = _err("MissingIntercept", 9)
+NoGenesAfterFiltering = _err("NoGenesAfterFiltering", 10)
+InvalidBounds = _err("InvalidBounds", 11)
+InvalidDispersion = _err("InvalidDispersion", 12)
+Gpu = _err("Gpu", 13)
+_BY_CODE = {c.code: c for c in (EmptyCounts, InvalidArgument, MustBePositive, LengthMismatch,
+                                ShapeMismatch, MalformedSparse, SubjectsNotContiguous,
+                                TooFewSubjects, MissingIntercept, NoGenesAfterFiltering,
+                                InvalidBounds, InvalidDispersion, Gpu)}
+
+
+def _raise(code, err):
+    msg = err.value.decode(errors="replace") if hasattr(err, "value") else str(err)
+    raise _BY_CODE.get(code, EdgeError)(msg)
+
+
+# ---------------------------------------------------------------------------
+class _Params(C.Structure):
+    _fields_ = [("min_subject", C.c_double), ("min_cell", C.c_double),
+                ("max_subject", C.c_double), ("max_cell", C.c_double),
+                ("method", C.c_int32), ("reml", C.c_int32),
+                ("cutoff_cell", C.c_double), ("kappa", C.c_double),
+                ("cpc", C.c_double), ("mincp", C.c_uint64),
+                ("min_subjects", C.c_uint64), ("eps", C.c_double)]
+
+
+class _Fit(C.Structure):
+    _fields_ = [("n_genes", C.c_size_t), ("n_coef", C.c_size_t),
+                ("coefficients", _dp), ("covariance", _dp), ("se", _dp),
+                ("subject_overdispersion", _dp), ("cell_overdispersion", _dp),
+                ("convergence", C.POINTER(C.c_int32)),
+                ("sigma_at_bound", C.POINTER(C.c_uint8)),
+                ("gene_index", _u64p)]
+
+
+class _Options(C.Structure):
+    _fields_ = [("engine", C.c_int32), ("learned", C.c_int32),
+                ("mlp_weights", _fp), ("wg_size", C.c_int32),
+                ("device", C.c_int32), ("host_threads", C.c_int32)]
+
+
+class RunStats(C.Structure):
+    """nebula_run_stats_t: per-phase wall clock and device counters."""
+    _fields_ = [("t_setup", C.c_double), ("t_stage_one", C.c_double),
+                ("t_stage_two", C.c_double), ("t_stage_three", C.c_double),
+                ("t_total", C.c_double), ("t_upload", C.c_double),
+                ("genes_searched", C.c_uint64), ("evaluations", C.c_uint64),
+                ("rounds", C.c_uint64), ("sweeps_f32", C.c_uint64),
+                ("sweeps_f64", C.c_uint64), ("cell_sweeps", C.c_uint64),
+                ("nnz_sweeps", C.c_uint64), ("warm_fallbacks", C.c_uint64),
+                ("fit_kernel_ms", C.c_double), ("fit_kernel_launches", C.c_uint64),
+                ("cell_sweeps_f64", C.c_uint64), ("nnz_sweeps_f64", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class _Request(C.Structure):
+    _fields_ = [("gene", C.c_uint32), ("ord", C.c_uint32), ("subject", C.c_double),
+                ("cell", C.c_double), ("beta_init", _dp), ("log_w_init", _dp)]
+
+
+class _Reply(C.Structure):
+    _fields_ = [("log_likelihood", C.c_double), ("log_likelihood_prev", C.c_double),
+                ("log_det", C.c_double), ("second_order", C.c_double),
+                ("convergence", C.c_int32), ("iterations", C.c_int32),
+                ("backtracks", C.c_int32), ("finish_steps", C.c_int32),
+                ("beta", _dp), ("log_w", _dp), ("information", _dp)]
+
+
+OBJECTIVE_FN = C.CFUNCTYPE(C.c_double, _dp, C.c_void_p)
+
+_lib = None
+
+
+def lib():
+    """Loads libnebula_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libnebula_hip.so not built at {LIB_PATH}; "
+                               "run python edge-rs_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        L.nebula_params_default.argtypes = [C.POINTER(_Params)]
+        L.nebula_hip_options_default.argtypes = [C.POINTER(_Options)]
+        L.nebula_sparse_hip_ex.restype = C.c_int
+        L.nebula_sparse_hip_ex.argtypes = [C.c_size_t, C.c_size_t, _u32p, _u32p, _dp, _u64p, _dp,
+                                           C.c_size_t, _dp, C.POINTER(_Params),
+                                           C.POINTER(_Options), C.POINTER(_Fit),
+                                           C.POINTER(RunStats), C.c_char_p, C.c_size_t]
+        L.nebula_sparse_hip.restype = C.c_int
+        L.nebula_sparse_hip.argtypes = [C.c_size_t, C.c_size_t, _u32p, _u32p, _dp, _u64p, _dp,
+                                        C.c_size_t, _dp, C.POINTER(_Params), C.POINTER(_Fit),
+                                        C.c_char_p, C.c_size_t]
+        L.nebula_fit_free.argtypes = [C.POINTER(_Fit)]
+        L.nh_resident_upload.restype = C.c_int
+        L.nh_resident_upload.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t, _dp, _dp, _u64p,
+                                         C.c_size_t, _u64p, _u32p, _dp, C.c_int32,
+                                         C.POINTER(C.c_void_p), C.c_char_p, C.c_size_t]
+        L.nh_resident_solve.restype = C.c_int
+        L.nh_resident_solve.argtypes = [C.c_void_p, C.POINTER(_Request), C.c_size_t, C.c_double,
+                                        C.c_int32, C.POINTER(_Reply), C.c_char_p, C.c_size_t]
+        L.nh_resident_free.argtypes = [C.c_void_p]
+        L.nh_bobyqa.restype = C.c_int
+        L.nh_bobyqa.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_int, OBJECTIVE_FN,
+                                C.c_void_p, _dp, _dp, C.POINTER(C.c_int)]
+        L.nh_bobyqa_device_trace.restype = C.c_int
+        L.nh_bobyqa_device_trace.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_int,
+                                             C.POINTER(C.c_int), C.c_char_p, C.c_size_t]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+# ---------------------------------------------------------------------------
+@dataclass
+class NebulaParams:
+    """NebulaParams (src/sc/nebula.rs:207-258) with the R package defaults."""
+    min: tuple = (1e-4, 1e-4)
+    max: tuple = (10.0, 1000.0)
+    method: str = "LN"
+    cutoff_cell: float = 20.0
+    kappa: float = 800.0
+    cpc: float = 0.005
+    mincp: int = 5
+    min_subjects: int = 0
+    reml: bool = False
+    eps: float = 1e-6
+
+    def _c(self):
+        return _Params(self.min[0], self.min[1], self.max[0], self.max[1],
+                       1 if self.method == "HL" else 0, int(self.reml), self.cutoff_cell,
+                       self.kappa, self.cpc, self.mincp, self.min_subjects, self.eps)
+
+
+@dataclass
+class CompressedSparse:
+    """Gene-major CSR counts (CompressedSparse<f64>, src/utils/sparse.rs:58-69)."""
+    data: np.ndarray
+    indices: np.ndarray
+    indptr: np.ndarray
+    shape: tuple
+    format: str = "csr"
+
+    @classmethod
+    def from_dense(cls, counts):
+        counts = np.asarray(counts, dtype=np.float64)
+        indptr = [0]
+        indices, data = [], []
+        for row in counts:
+            nz = np.nonzero(row != 0)[0]
+            indices.extend(nz.tolist())
+            data.extend(row[nz].tolist())
+            indptr.append(len(data))
+        return cls(np.array(data, dtype=np.float64), np.array(indices, dtype=np.uint32),
+                   np.array(indptr, dtype=np.uint32), counts.shape)
+
+
+@dataclass
+class NebulaFit:
+    """NebulaFit (src/sc/nebula.rs:301-344)."""
+    coefficients: np.ndarray
+    covariance: np.ndarray
+    se: np.ndarray
+    subject_overdispersion: np.ndarray
+    cell_overdispersion: np.ndarray
+    convergence: np.ndarray
+    sigma_at_bound: np.ndarray
+    gene_index: np.ndarray
+    n_coef: int
+    stats: dict = field(default_factory=dict)
+
+
+def nebula_sparse_hip(counts: CompressedSparse, subject_id, design, n_coef: int,
+                      offset=None, params: Optional[NebulaParams] = None,
+                      engine: int = ENGINE_BATCHED, learned: bool = False,
+                      mlp_weights=None, wg_size: int = 0, device: int = 0,
+                      host_threads: int = 0) -> NebulaFit:
+    """NEBULA with stages one to three on the GPU (edge-rs nebula_sparse_gpu)."""
+    if counts.format != "csr":
+        raise MalformedSparse("nebula_sparse needs gene-major counts in CSR form.")
+    n_genes, n_cells = counts.shape
+    design = np.ascontiguousarray(design, dtype=np.float64)
+    if design.size != n_cells * n_coef:
+        raise ShapeMismatch(f"design has {design.size} values, expected {n_cells}x{n_coef}")
+    subject_id = np.ascontiguousarray(subject_id, dtype=np.uint64)
+    if subject_id.size != n_cells:
+        raise LengthMismatch(f"subject_id has {subject_id.size} entries, expected {n_cells}")
+    if offset is not None and np.asarray(offset).size != n_cells:
+        raise LengthMismatch(f"offset has {np.asarray(offset).size} entries, expected {n_cells}")
+    indptr = np.ascontiguousarray(counts.indptr, dtype=np.uint32)
+    indices = np.ascontiguousarray(counts.indices, dtype=np.uint32)
+    data = np.ascontiguousarray(counts.data, dtype=np.float64)
+    off = None
+    offp = None
+    if offset is not None:
+        off, offp = _d(offset)
+    p = (params or NebulaParams())._c()
+    o = _Options()
+    lib().nebula_hip_options_default(C.byref(o))
+    o.engine = engine
+    o.learned = int(learned)
+    w = None
+    if mlp_weights is not None:
+        w = np.ascontiguousarray(mlp_weights, dtype=np.float32)
+        o.mlp_weights = w.ctypes.data_as(_fp)
+    o.wg_size = wg_size
+    o.device = device
+    o.host_threads = host_threads
+    fit = _Fit()
+    st = RunStats()
+    err = C.create_string_buffer(1024)
+    rc = lib().nebula_sparse_hip_ex(n_genes, n_cells, indptr.ctypes.data_as(_u32p),
+                                    indices.ctypes.data_as(_u32p), data.ctypes.data_as(_dp),
+                                    subject_id.ctypes.data_as(_u64p), design.ctypes.data_as(_dp),
+                                    n_coef, offp, C.byref(p), C.byref(o), C.byref(fit),
+                                    C.byref(st), err, 1024)
+    if rc != 0:
+        _raise(rc, err)
+    g, nb = fit.n_genes, fit.n_coef
+    pk = nb * (nb + 1) // 2
+    arr = np.ctypeslib.as_array
+    out = NebulaFit(
+        coefficients=arr(fit.coefficients, (g * nb,)).reshape(g, nb).copy(),
+        covariance=arr(fit.covariance, (g * pk,)).reshape(g, pk).copy(),
+        se=arr(fit.se, (g * nb,)).reshape(g, nb).copy(),
+        subject_overdispersion=arr(fit.subject_overdispersion, (g,)).copy(),
+        cell_overdispersion=arr(fit.cell_overdispersion, (g,)).copy(),
+        convergence=arr(fit.convergence, (g,)).copy(),
+        sigma_at_bound=arr(fit.sigma_at_bound, (g,)).astype(bool),
+        gene_index=arr(fit.gene_index, (g,)).astype(np.int64),
+        n_coef=nb, stats=st.as_dict())
+    lib().nebula_fit_free(C.byref(fit))
+    return out
+
+
+# ---------------------------------------------------------------------------
+class ResidentBatch:
+    """Device-resident design, offsets, subjects and per-gene counts; batches
+    of penalised fits on them (edge-rs ResidentBatch upload / solve)."""
+
+    SOLVE_F32_FINISH = 0
+    SOLVE_F64 = 1
+
+    def __init__(self, design, log_offset, subject_start, gene_ptr, cells, counts, device=0):
+        self.design, dp = _d(design)
+        self.n_cells = len(log_offset)
+        self.n_coef = self.design.size // self.n_cells
+        self.k = len(subject_start) - 1
+        lo, lop = _d(log_offset)
+        ss = np.ascontiguousarray(subject_start, dtype=np.uint64)
+        gp = np.ascontiguousarray(gene_ptr, dtype=np.uint64)
+        ce = np.ascontiguousarray(cells, dtype=np.uint32)
+        y, yp = _d(counts)
+        self.n_genes = len(gp) - 1
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        rc = lib().nh_resident_upload(self.n_cells, self.n_coef, self.k, dp, lop,
+                                      ss.ctypes.data_as(_u64p), self.n_genes,
+                                      gp.ctypes.data_as(_u64p), ce.ctypes.data_as(_u32p), yp,
+                                      device, C.byref(h), err, 512)
+        if rc != 0:
+            _raise(rc, err)
+        self._h = h
+
+    def solve(self, genes, subject, cell, beta_init, log_w_init=None, ord=1, eps=1e-6,
+              mode=SOLVE_F32_FINISH):
+        """One penalised fit per request; returns a dict of arrays."""
+        n = len(genes)
+        nb, k = self.n_coef, self.k
+        bi = np.ascontiguousarray(np.broadcast_to(np.asarray(beta_init, dtype=np.float64),
+                                                  (n, nb)))
+        lw = None
+        if log_w_init is not None:
+            lw = np.ascontiguousarray(np.broadcast_to(np.asarray(log_w_init, dtype=np.float64),
+                                                      (n, k)))
+        ords = np.broadcast_to(np.asarray(ord), (n,))
+        subject = np.broadcast_to(np.asarray(subject, dtype=np.float64), (n,))
+        cell = np.broadcast_to(np.asarray(cell, dtype=np.float64), (n,))
+        reqs = (_Request * n)()
+        beta = np.zeros((n, nb))
+        logw = np.zeros((n, k))
+        info = np.zeros((n, nb * nb))
+        reps = (_Reply * n)()
+        for q in range(n):
+            reqs[q].gene = int(genes[q])
+            reqs[q].ord = int(ords[q])
+            reqs[q].subject = float(subject[q])
+            reqs[q].cell = float(cell[q])
+            reqs[q].beta_init = bi[q].ctypes.data_as(_dp)
+            reqs[q].log_w_init = lw[q].ctypes.data_as(_dp) if lw is not None else None
+            reps[q].beta = beta[q].ctypes.data_as(_dp)
+            reps[q].log_w = logw[q].ctypes.data_as(_dp)
+            reps[q].information = info[q].ctypes.data_as(_dp)
+        err = C.create_string_buffer(512)
+        rc = lib().nh_resident_solve(self._h, reqs, n, eps, mode, reps, err, 512)
+        if rc != 0:
+            _raise(rc, err)
+        return dict(
+            beta=beta, log_w=logw, information=info,
+            log_likelihood=np.array([r.log_likelihood for r in reps]),
+            log_likelihood_prev=np.array([r.log_likelihood_prev for r in reps]),
+            log_det=np.array([r.log_det for r in reps]),
+            second_order=np.array([r.second_order for r in reps]),
+            iterations=np.array([r.iterations for r in reps]),
+            backtracks=np.array([r.backtracks for r in reps]),
+            finish_steps=np.array([r.finish_steps for r in reps]))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nh_resident_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def bobyqa(f, x0, lower, upper, xtol_rel=1e-6, maxeval=1000):
+    """The library's host BOBYQA (NLopt 2.7.1 semantics) on a Python objective."""
+    n = len(x0)
+    x0, x0p = _d(x0)
+    lo, lop = _d(lower)
+    hi, hip = _d(upper)
+
+    @OBJECTIVE_FN
+    def cb(xp, _ctx):
+        return float(f(np.ctypeslib.as_array(xp, (n,)).copy()))
+
+    xo = np.zeros(n)
+    fo = C.c_double()
+    ev = C.c_int()
+    status = lib().nh_bobyqa(n, x0p, lop, hip, xtol_rel, maxeval, cb, None,
+                             xo.ctypes.data_as(_dp), C.byref(fo), C.byref(ev))
+    return xo, fo.value, ev.value, status
+
+
+def bobyqa_device_trace(problem, x0, lower, upper, max_trace=2000):
+    """Runs the device BOBYQA on built-in problem (0 rosen, 1 corner,
+    2 ripple, 3 valley, 4 chain3); returns (trace rows, status)."""
+    n = len(x0)
+    x0, x0p = _d(x0)
+    lo, lop = _d(lower)
+    hi, hip = _d(upper)
+    tr = np.zeros(max_trace * (n + 1))
+    ev = C.c_int()
+    err = C.create_string_buffer(512)
+    status = lib().nh_bobyqa_device_trace(problem, n, x0p, lop, hip, tr.ctypes.data_as(_dp),
+                                          max_trace, C.byref(ev), err, 512)
+    if status <= -100:
+        _raise(-status - 100, err)
+    rows = min(ev.value, max_trace)
+    return tr[: rows * (n + 1)].reshape(rows, n + 1), status
+
+
+# ---------------------------------------------------------------------------
+class Session:
+    """nebula_sparse_hip split at the device seam: set-up, upload and stage one
+    on construction; stage_two() runs the search with inputs resident on the
+    device (repeatable, timeable); finish() runs stage three."""
+
+    def __init__(self, counts: CompressedSparse, subject_id, design, n_coef, offset=None,
+                 params: Optional[NebulaParams] = None, wg_size=0, device=0, host_threads=0):
+        L = lib()
+        if not hasattr(L, "_session_bound"):
+            L.nh_session_create.restype = C.c_int
+            L.nh_session_create.argtypes = [C.c_size_t, C.c_size_t, _u32p, _u32p, _dp, _u64p, _dp,
+                                            C.c_size_t, _dp, C.POINTER(_Params),
+                                            C.POINTER(_Options), C.POINTER(C.c_void_p),
+                                            C.c_char_p, C.c_size_t]
+            L.nh_session_genes.restype = C.c_size_t
+            L.nh_session_genes.argtypes = [C.c_void_p]
+            L.nh_session_stage_one.restype = C.c_int
+            L.nh_session_stage_one.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int32)]
+            L.nh_session_stage_two.restype = C.c_int
+            L.nh_session_stage_two.argtypes = [C.c_void_p, C.c_int32, _dp, _dp,
+                                               C.POINTER(C.c_int32), C.POINTER(RunStats),
+                                               C.c_char_p, C.c_size_t]
+            L.nh_session_finish.restype = C.c_int
+            L.nh_session_finish.argtypes = [C.c_void_p, C.POINTER(_Fit), _dp, C.c_char_p,
+                                            C.c_size_t]
+            L.nh_session_free.argtypes = [C.c_void_p]
+            L._session_bound = True
+        n_genes, n_cells = counts.shape
+        self._keep = [np.ascontiguousarray(counts.indptr, dtype=np.uint32),
+                      np.ascontiguousarray(counts.indices, dtype=np.uint32),
+                      np.ascontiguousarray(counts.data, dtype=np.float64),
+                      np.ascontiguousarray(subject_id, dtype=np.uint64),
+                      np.ascontiguousarray(design, dtype=np.float64)]
+        ip, ix, dv, sid, de = self._keep
+        off = None
+        offp = None
+        if offset is not None:
+            off, offp = _d(offset)
+        p = (params or NebulaParams())._c()
+        o = _Options()
+        lib().nebula_hip_options_default(C.byref(o))
+        o.wg_size = wg_size
+        o.device = device
+        o.host_threads = host_threads
+        h = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        rc = L.nh_session_create(n_genes, n_cells, ip.ctypes.data_as(_u32p),
+                                 ix.ctypes.data_as(_u32p), dv.ctypes.data_as(_dp),
+                                 sid.ctypes.data_as(_u64p), de.ctypes.data_as(_dp), n_coef, offp,
+                                 C.byref(p), C.byref(o), C.byref(h), err, 1024)
+        if rc != 0:
+            _raise(rc, err)
+        self._h = h
+        self.n_kept = L.nh_session_genes(h)
+
+    def stage_one(self):
+        sg = np.zeros((self.n_kept, 2))
+        ords = np.zeros(self.n_kept, dtype=np.int32)
+        lib().nh_session_stage_one(self._h, sg.ctypes.data_as(_dp),
+                                   ords.ctypes.data_as(C.POINTER(C.c_int32)))
+        return sg, ords
+
+    def stage_two(self, engine=ENGINE_BATCHED):
+        mins = np.zeros((self.n_kept, 2))
+        obj = np.zeros(self.n_kept)
+        ev = np.zeros(self.n_kept, dtype=np.int32)
+        st = RunStats()
+        err = C.create_string_buffer(1024)
+        rc = lib().nh_session_stage_two(self._h, engine, mins.ctypes.data_as(_dp),
+                                        obj.ctypes.data_as(_dp),
+                                        ev.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(st),
+                                        err, 1024)
+        if rc != 0:
+            _raise(rc, err)
+        return dict(minimisers=mins, objectives=obj, evaluations=ev, stats=st.as_dict())
+
+    def finish(self):
+        fit = _Fit()
+        t3 = C.c_double()
+        err = C.create_string_buffer(1024)
+        rc = lib().nh_session_finish(self._h, C.byref(fit), C.byref(t3), err, 1024)
+        if rc != 0:
+            _raise(rc, err)
+        g, nb = fit.n_genes, fit.n_coef
+        pk = nb * (nb + 1) // 2
+        arr = np.ctypeslib.as_array
+        out = NebulaFit(
+            coefficients=arr(fit.coefficients, (g * nb,)).reshape(g, nb).copy(),
+            covariance=arr(fit.covariance, (g * pk,)).reshape(g, pk).copy(),
+            se=arr(fit.se, (g * nb,)).reshape(g, nb).copy(),
+            subject_overdispersion=arr(fit.subject_overdispersion, (g,)).copy(),
+            cell_overdispersion=arr(fit.cell_overdispersion, (g,)).copy(),
+            convergence=arr(fit.convergence, (g,)).copy(),
+            sigma_at_bound=arr(fit.sigma_at_bound, (g,)).astype(bool),
+            gene_index=arr(fit.gene_index, (g,)).astype(np.int64),
+            n_coef=nb, stats={"t_stage_three": t3.value})
+        lib().nebula_fit_free(C.byref(fit))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nh_session_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

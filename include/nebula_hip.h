@@ -121,11 +121,13 @@ typedef struct {
   uint64_t rounds;           /* launches of the stage-two fit kernel */
   uint64_t sweeps_f32;       /* passes over a gene's cells, FP32 */
   uint64_t sweeps_f64;       /* passes over a gene's cells, FP64 */
-  uint64_t cell_sweeps;      /* sum over sweeps of the gene's cell count */
-  uint64_t nnz_sweeps;       /* sum over sweeps of the gene's positive counts */
+  uint64_t cell_sweeps;      /* sum over FP32 sweeps of the gene's cell count */
+  uint64_t nnz_sweeps;       /* sum over FP32 sweeps of its positive counts */
   uint64_t warm_fallbacks;   /* learned starts rejected by the log-lik rule */
   double fit_kernel_ms;      /* device time of the stage-two kernels (events) */
   uint64_t fit_kernel_launches;
+  uint64_t cell_sweeps_f64;  /* the same over FP64 sweeps */
+  uint64_t nnz_sweeps_f64;
 } nebula_run_stats_t;
 
 void nebula_params_default(nebula_params_t* params);
@@ -156,6 +158,36 @@ int nebula_sparse_hip_ex(size_t n_genes, size_t n_cells, const uint32_t* indptr,
                          char* err, size_t err_len);
 
 void nebula_fit_free(nebula_fit_t* fit);
+
+/* ---------------------------------------------------------------------------
+ * Session API: nebula_sparse_hip split at edge-rs's device seam
+ * (nebula_sparse_with's `fit` closure, src/sc/nebula.rs:494-511, and
+ * search_all, src/gpu/stage_two.rs:362).  create = validation, set-up,
+ * upload and stage one; stage_two can then run (and be timed) repeatedly with
+ * the inputs resident on the device; finish = stage three into a NebulaFit.
+ * ------------------------------------------------------------------------- */
+typedef struct nh_session nh_session_t;
+
+int nh_session_create(size_t n_genes, size_t n_cells, const uint32_t* indptr,
+                      const uint32_t* indices, const double* data,
+                      const uint64_t* subject_id, const double* design,
+                      size_t n_coef, const double* offset,
+                      const nebula_params_t* params,
+                      const nebula_hip_options_t* options, nh_session_t** out,
+                      char* err, size_t err_len);
+/* Genes that survived the filter (the rows of every per-gene output). */
+size_t nh_session_genes(const nh_session_t* s);
+/* Stage one's variance components (kept x 2: sigma, gamma) and Laplace order. */
+int nh_session_stage_one(const nh_session_t* s, double* sigma_gamma, int32_t* ord);
+/* Runs stage two with engine NH_ENGINE_*.  minimisers: kept x 2 (subject,
+ * cell), objectives: kept (the profile objective at the minimiser),
+ * evaluations: kept; any may be NULL. */
+int nh_session_stage_two(nh_session_t* s, int32_t engine, double* minimisers,
+                         double* objectives, int32_t* evaluations,
+                         nebula_run_stats_t* stats, char* err, size_t err_len);
+int nh_session_finish(nh_session_t* s, nebula_fit_t* out, double* t_stage_three,
+                      char* err, size_t err_len);
+void nh_session_free(nh_session_t* s);
 
 /* ---------------------------------------------------------------------------
  * Request-level batch API: one penalised fit (opt_pml) per request, all on the
@@ -225,8 +257,8 @@ int nh_bobyqa(int n, const double* x0, const double* lower, const double* upper,
 
 /* The same stepper run on the device, objective = a built-in test function
  * (0 rosen, 1 corner, 2 ripple, 3 valley, 4 chain3) evaluated on the device.
- * trace: max_trace * (n + 1) doubles.  Returns the NLopt status or a negative
- * NH_ERR_* on a device error (then err holds the message). */
+ * trace: max_trace * (n + 1) doubles.  Returns the NLopt status, or
+ * -100 - NH_ERR_* on an error (then err holds the message). */
 int nh_bobyqa_device_trace(int problem, int n, const double* x0,
                            const double* lower, const double* upper,
                            double* trace, int max_trace, int* evals_out,

@@ -96,6 +96,36 @@ int oracle_nebula_sparse(std::size_t n_genes, std::size_t n_cells, const std::ui
   }
 }
 
+int oracle_profile_objective(std::size_t n_genes, std::size_t n_cells,
+                             const std::uint32_t* indptr, const std::uint32_t* indices,
+                             const double* data, const std::uint64_t* subject_id,
+                             const double* design, std::size_t n_coef, const double* offset,
+                             const nebula_params_t* p, std::size_t n, const std::uint64_t* gene,
+                             const double* point, double* out, char* err, std::size_t err_len) {
+  Params prm;
+  if (p) {
+    prm.min_subject = p->min_subject;
+    prm.min_cell = p->min_cell;
+    prm.max_subject = p->max_subject;
+    prm.max_cell = p->max_cell;
+    prm.method = p->method;
+    prm.cutoff_cell = p->cutoff_cell;
+    prm.kappa = p->kappa;
+    prm.cpc = p->cpc;
+    prm.mincp = p->mincp;
+    prm.min_subjects = p->min_subjects;
+    prm.eps = p->eps;
+  }
+  try {
+    profile_objective(n_genes, n_cells, indptr, indices, data, subject_id, design, n_coef, offset,
+                      prm, n, gene, point, out);
+    return 0;
+  } catch (const Error& e) {
+    put_err(err, err_len, e.message);
+    return e.code;
+  }
+}
+
 void oracle_fit_free(nebula_fit_t* f) {
   std::free(f->coefficients);
   std::free(f->covariance);

@@ -504,6 +504,36 @@ bool tables_apply(const std::vector<double>& centred, std::size_t n_cells, std::
 
 }  // namespace
 
+namespace {
+// When set, nebula_sparse evaluates the profile objective at these points
+// instead of fitting: (kept gene index, subject, cell) -> value.
+struct ProfileProbe {
+  std::size_t n;
+  const std::uint64_t* gene;
+  const double* point;
+  double* out;
+};
+thread_local ProfileProbe* g_probe = nullptr;
+}  // namespace
+
+void profile_objective(std::size_t n_genes, std::size_t n_cells, const std::uint32_t* indptr,
+                       const std::uint32_t* indices, const double* data,
+                       const std::uint64_t* subject_id, const double* design,
+                       std::size_t n_coef, const double* offset, const Params& params,
+                       std::size_t n, const std::uint64_t* gene, const double* point,
+                       double* out) {
+  ProfileProbe probe{n, gene, point, out};
+  g_probe = &probe;
+  try {
+    nebula_sparse(n_genes, n_cells, indptr, indices, data, subject_id, design, n_coef, offset,
+                  params, 1, nullptr, nullptr);
+  } catch (...) {
+    g_probe = nullptr;
+    throw;
+  }
+  g_probe = nullptr;
+}
+
 Fit nebula_sparse(std::size_t n_genes, std::size_t n_cells, const std::uint32_t* indptr,
                   const std::uint32_t* indices, const double* data,
                   const std::uint64_t* subject_id, const double* design,
@@ -650,6 +680,37 @@ Fit nebula_sparse(std::size_t n_genes, std::size_t n_cells, const std::uint32_t*
     if (pooled) kept.push_back(g);
   }
   if (kept.empty()) fail(10, "no genes after filtering");
+
+  if (g_probe) {
+    ProfileProbe* pr = g_probe;
+    for (std::size_t q = 0; q < pr->n; ++q) {
+      const std::size_t idx = pr->gene[q];
+      if (idx >= kept.size()) fail(2, "probe gene out of range");
+      const std::size_t g = kept[idx];
+      GeneCounts gc;
+      double sum = 0.0;
+      for (std::uint32_t i = indptr[g]; i < indptr[g + 1]; ++i)
+        if (data[i] > 0.0) {
+          gc.cells.push_back(indices[i]);
+          gc.counts.push_back(data[i]);
+          sum += data[i];
+          if (data[i] == 1.0) ++gc.n_one;
+          else if (data[i] == 2.0) ++gc.n_two;
+        }
+      gc.mean_count = sum / (double)n_cells;
+      const double* tot = &totals[g * k];
+      PmlData pd{sh.design, sh.log_offset, n_cells, n_coef, gc.counts.data(), gc.cells.data(),
+                 gc.counts.size(), fid.data(), k, tot};
+      std::vector<double> bs(n_coef, 0.0);
+      bs[sh.intercept] = std::log(gc.mean_count) - sh.log_mean_offset;
+      Objective obj{&pd, &bs, sh.intercept, PmlParams{}, (double)n_cells, (double)k,
+                    &gc.counts, (double)gc.counts.size(), (double)gc.n_one, (double)gc.n_two};
+      obj.params.eps = params.eps;
+      obj.params.ord = gc.mean_count * sh.cells_per_subject < kHighOrderCutoff ? 3 : 1;
+      pr->out[q] = obj.value(pr->point + 2 * q);
+    }
+    return Fit{};
+  }
 
   std::vector<Outcome> outcomes(kept.size());
   if (stage_two) stage_two->assign(kept.size(), StageTwoInfo{});

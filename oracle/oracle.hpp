@@ -163,4 +163,14 @@ Fit nebula_sparse(std::size_t n_genes, std::size_t n_cells,
                   std::vector<StageTwoInfo>* stage_two,
                   double* t_stage_two_out);
 
+// The profile objective (VarianceObjective::value, nebula.rs:1489) at n
+// points (subject, cell) for kept genes gene[q], after the same set-up as
+// nebula_sparse.
+void profile_objective(std::size_t n_genes, std::size_t n_cells, const std::uint32_t* indptr,
+                       const std::uint32_t* indices, const double* data,
+                       const std::uint64_t* subject_id, const double* design,
+                       std::size_t n_coef, const double* offset, const Params& params,
+                       std::size_t n, const std::uint64_t* gene, const double* point,
+                       double* out);
+
 }  // namespace oracle

@@ -241,3 +241,34 @@ def nebula_sparse(indptr, indices, data, n_cells, subject_id, design, n_coef, of
         out["stage_two"] = info[: 8 * g].reshape(g, 8).copy()
     lib().oracle_fit_free(C.byref(fit))
     return out
+
+
+def profile_objective(indptr, indices, data, n_cells, subject_id, design, n_coef, offset, prm,
+                      genes, points):
+    """The f64 profile objective (nebula's pql_ll, -loglik) at (subject, cell)
+    points for kept-gene indices `genes`."""
+    n_genes = len(indptr) - 1
+    ip, ipp = _u32(indptr)
+    ix, ixp = _u32(indices)
+    dv, dvp = _d(data)
+    sid, sidp = _u64(subject_id)
+    de, dep = _d(design)
+    off = None
+    offp = None
+    if offset is not None:
+        off, offp = _d(offset)
+    g, gp = _u64(genes)
+    pts, ptsp = _d(np.asarray(points, dtype=np.float64).reshape(-1, 2))
+    out = np.zeros(len(g))
+    err = C.create_string_buffer(512)
+    L = lib()
+    L.oracle_profile_objective.restype = C.c_int
+    L.oracle_profile_objective.argtypes = [C.c_size_t, C.c_size_t, _u32p, _u32p, _dp, _u64p, _dp,
+                                           C.c_size_t, _dp, C.POINTER(NebulaParams), C.c_size_t,
+                                           _u64p, _dp, _dp, C.c_char_p, C.c_size_t]
+    rc = L.oracle_profile_objective(n_genes, n_cells, ipp, ixp, dvp, sidp, dep, n_coef, offp,
+                                    C.byref(prm), len(g), gp, ptsp, out.ctypes.data_as(_dp),
+                                    err, 512)
+    if rc != 0:
+        raise OracleError(rc, err.value.decode())
+    return out

@@ -1,0 +1,233 @@
+"""HIP path against the oracle and the reference's golden data, through the
+C ABI (libnebula_hip.so).  Marked gpu: these run on an MI355X."""
+import numpy as np
+import pytest
+
+import oracle
+from fixtures import SC_DATASETS, assert_close, dense_to_csr, golden8, load_sc, read_matrix
+import nebula_hip as nh
+from nebula_hip import synth
+
+pytestmark = pytest.mark.gpu
+
+PROBLEMS = {"rosen": (0, [-1.2, 1.0], [-2.0, -1.0], [2.0, 3.0], 4),
+            "rosen_edge": (0, [0.4001, 0.2], [0.4, 0.2], [2.0, 2.0], 4),
+            "corner": (1, [1.0, 1.0], [0.0, 0.0], [2.0, 2.0], 4),
+            "rescue": (2, [-1.0, 0.25], [-2.0, -2.0], [2.0, 2.0], 4),
+            "roundoff": (3, [0.0, 0.0], [-2.0, -2.0], [2.0, 2.0], -4),
+            "chain3": (4, [0.5, 2.0, -1.0], [-3.0] * 3, [3.0] * 3, 4)}
+
+
+@pytest.mark.parametrize("tag", list(PROBLEMS))
+def test_device_bobyqa_retraces_nloptr(tag):
+    """The stepper running on the device asks for nloptr's points bit for bit
+    (traces from src/numeric/bobyqa.rs:2947-2970's fixtures)."""
+    prob, x0, lo, hi, status = PROBLEMS[tag]
+    want = read_matrix(f"bobyqa_{tag}.csv")
+    got, st = nh.bobyqa_device_trace(prob, x0, lo, hi, max_trace=len(want) + 10)
+    assert st == status
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)
+
+
+# pml fixture A (src/sc/pml.rs:2151-2180)
+DESIGN = np.array([1, 0, 1, 1, 1, 0, 1, 1, 1, 1, 1, 0, 1, 1, 1, 0, 1, 1, 1, 0, 1, 1, 1, 1],
+                  dtype=float).reshape(12, 2)
+OFFSET = np.array([0.25, -0.5, 0.75, 0.0, 0.5, -0.25, 0.125, 0.375, -0.125, 0.25, 0.0, -0.375])
+START = np.array([0, 4, 9, 12], dtype=np.uint64)
+COUNTS = np.array([3.0, 5.0, 2.0, 4.0, 1.0, 7.0, 2.0, 6.0])
+CELLS = np.array([0, 2, 3, 5, 6, 7, 9, 10], dtype=np.uint32)
+
+
+def test_device_f64_fit_reproduces_nebula_values():
+    """The device opt_pml in f64 (stage three's mode) follows opt_pml's control
+    flow exactly: nebula 1.5.8's optimum at 1e-11 and the same iteration count."""
+    rb = nh.ResidentBatch(DESIGN, OFFSET, START, [0, len(COUNTS)], CELLS, COUNTS)
+    r = rb.solve([0, 0, 0], [0.25, 0.25, 8.0], [2.0, 2.0, 0.001],
+                 [[0.5, -0.25], [90.0, 0.0], [15.0, 5.0]], mode=nh.ResidentBatch.SOLVE_F64)
+    np.testing.assert_allclose(r["beta"][0], [1.1055943907637618, -0.98348970166026206], rtol=1e-11)
+    np.testing.assert_allclose(r["log_w"][0], [-0.025429869309964156, 0.022419629678247836,
+                                               0.33781545877640862], rtol=1e-11)
+    np.testing.assert_allclose(r["information"][0], [5.4708370512237092, 2.6193993778249318,
+                                                     2.6193993778249318, 4.0223986651682804],
+                               rtol=1e-11)
+    np.testing.assert_allclose(r["log_likelihood"][0], -60.378642466146786, rtol=1e-11)
+    np.testing.assert_allclose(r["log_det"][0], 6.0421923783970968, rtol=1e-11)
+    assert r["iterations"][0] == 4
+    np.testing.assert_allclose(r["beta"][1], [1.1055943759446161, -0.98348968952057669], rtol=1e-11)
+    assert r["iterations"][1] == 9
+    np.testing.assert_allclose(r["beta"][2], [4.9609375, -4.9609375], rtol=1e-11)
+    assert (r["iterations"][2], r["backtracks"][2]) == (3, 11)
+
+
+def synthetic_gene_batch(n_cells=6000, k=12, p=4, n_genes=6, seed=7):
+    w = synth.generate("t", n_genes, n_cells, k, p, "dirichlet2", ("normal", -1.0, 0.5), seed)
+    fid = np.concatenate([[0], np.cumsum(w.truth["sizes"])]).astype(np.uint64)
+    # centre the design as the driver does
+    d = w.design.copy()
+    for j in range(1, p):
+        d[:, j] = (d[:, j] - d[:, j].mean()) / d[:, j].std()
+    return w, d, np.log(w.offset), fid
+
+
+def test_fp32_fit_with_fp64_finish_matches_oracle():
+    """Stage two's device fit: FP32 Newton then the FP64 finish, against the
+    oracle's f64 opt_pml run to a tight tolerance (so its penultimate-iterate
+    log-determinant is the optimum's)."""
+    w, d, lo, fid = synthetic_gene_batch()
+    rb = nh.ResidentBatch(d, lo, fid, w.indptr.astype(np.uint64), w.indices, w.data)
+    genes = np.arange(w.n_genes)
+    subj = np.linspace(0.05, 0.6, w.n_genes)
+    cell = np.linspace(0.3, 4.0, w.n_genes)
+    beta0 = np.zeros((w.n_genes, w.n_coef))
+    beta0[:, 0] = -1.0
+    r = rb.solve(genes, subj, cell, beta0, mode=nh.ResidentBatch.SOLVE_F32_FINISH)
+    for g in genes:
+        a, b = int(w.indptr[g]), int(w.indptr[g + 1])
+        cells = w.indices[a:b].astype(np.uint64)
+        tot = np.zeros(len(fid) - 1)
+        np.add.at(tot, np.searchsorted(fid, cells, side="right") - 1, w.data[a:b])
+        o = oracle.opt_pml(d, lo, w.data[a:b], cells, fid, tot, beta0[g], subj[g], cell[g],
+                           eps=1e-13)
+        assert abs(r["log_likelihood"][g] - o["log_likelihood"]) <= 1e-10 * abs(o["log_likelihood"])
+        np.testing.assert_allclose(r["beta"][g], o["beta"], rtol=1e-6, atol=1e-8)
+        np.testing.assert_allclose(r["log_det"][g], o["log_det"], rtol=1e-8)
+        assert r["finish_steps"][g] >= 1
+
+
+def test_f64_mode_matches_oracle_iterations():
+    """Stage three's mode (f64 opt_pml, ord 3) against the oracle on 6000-cell
+    genes: same iteration count, log-likelihood to 1e-12, and the quantities
+    taken at the penultimate iterate (log-det, information) to 1e-9.
+
+    The last step's accept test is `likdif < 0`; when the last likdif is at
+    rounding level (1.8e-12 on gene 4 here) the two summation orders can
+    take different numbers of halvings, moving beta by a fraction of that
+    last step.  Where the backtrack counts differ, beta and the Laplace term
+    at the final point are held to 1e-6 instead of 1e-10."""
+    w, d, lo, fid = synthetic_gene_batch(seed=11)
+    rb = nh.ResidentBatch(d, lo, fid, w.indptr.astype(np.uint64), w.indices, w.data)
+    genes = np.arange(w.n_genes)
+    beta0 = np.zeros((w.n_genes, w.n_coef))
+    r = rb.solve(genes, 0.3, 1.5, beta0, mode=nh.ResidentBatch.SOLVE_F64, ord=3)
+    same = 0
+    for g in genes:
+        a, b = int(w.indptr[g]), int(w.indptr[g + 1])
+        cells = w.indices[a:b].astype(np.uint64)
+        tot = np.zeros(len(fid) - 1)
+        np.add.at(tot, np.searchsorted(fid, cells, side="right") - 1, w.data[a:b])
+        o = oracle.opt_pml(d, lo, w.data[a:b], cells, fid, tot, beta0[g], 0.3, 1.5, ord=3)
+        assert r["iterations"][g] == o["iterations"]
+        tie = r["backtracks"][g] != o["backtracks"]
+        if tie:
+            assert abs(o["log_likelihood"] - o["log_likelihood_prev"]) < 1e-9
+        else:
+            same += 1
+        rt = 1e-6 if tie else 1e-10
+        np.testing.assert_allclose(r["beta"][g], o["beta"], rtol=rt, atol=1e-12)
+        np.testing.assert_allclose(r["log_likelihood"][g], o["log_likelihood"], rtol=1e-12)
+        np.testing.assert_allclose(r["log_det"][g], o["log_det"], rtol=1e-10)
+        np.testing.assert_allclose(r["second_order"][g], o["second_order"],
+                                   rtol=1e-6 if tie else 1e-9)
+        np.testing.assert_allclose(r["information"][g], o["information"], rtol=1e-9, atol=1e-10)
+    assert same >= w.n_genes - 1
+
+
+ENGINES = {"plain": nh.ENGINE_PLAIN, "batched": nh.ENGINE_BATCHED}
+
+
+def run_hip(ip, ix, dv, n_cells, subject, design, n_coef, offset, params, engine):
+    counts = nh.CompressedSparse(dv, ix, ip, (len(ip) - 1, n_cells))
+    return nh.nebula_sparse_hip(counts, subject, design, n_coef, offset, params, engine=engine)
+
+
+def assert_objective_parity(ip, ix, dv, n_cells, subject, design, n_coef, offset, prm, fit,
+                            bound=1e-6):
+    """The stage-two criterion: the oracle's f64 profile objective at the HIP
+    minimiser and at the oracle's own minimiser agree to `bound` relative,
+    gene by gene, over the genes the oracle searched."""
+    ref = oracle.nebula_sparse(ip, ix, dv, n_cells, subject, design, n_coef, offset, prm,
+                               with_stage_two=True)
+    assert np.array_equal(fit.gene_index, ref["gene_index"])
+    s2 = ref["stage_two"]
+    g = np.nonzero(s2[:, 0] > 0)[0]
+    assert len(g) > 0
+    pts_hip = np.c_[fit.subject_overdispersion, 1.0 / fit.cell_overdispersion][g]
+    v_hip = oracle.profile_objective(ip, ix, dv, n_cells, subject, design, n_coef, offset, prm,
+                                     g, pts_hip)
+    v_cpu = oracle.profile_objective(ip, ix, dv, n_cells, subject, design, n_coef, offset, prm,
+                                     g, s2[g, 1:3])
+    assert np.all(np.isfinite(v_cpu))
+    rel = np.abs(v_hip - v_cpu) / np.abs(v_cpu)
+    assert np.all(np.isfinite(rel)) and rel.max() < bound, (rel.max(), int(g[np.argmax(rel)]))
+    return rel.max()
+
+
+@pytest.mark.parametrize("engine", list(ENGINES))
+def test_golden8_matches_r(engine):
+    """The 8-gene golden (src/sc/nebula.rs:2457-2560) under the GPU path's
+    gates (tests/e2e_nebula.rs:1010-1020), and the stage-two objective at the
+    oracle's precision."""
+    from test_oracle_nebula import GPU_TOLS
+    c, subject, d, offset = golden8()
+    ip, ix, dv = dense_to_csr(c)
+    fit = run_hip(ip, ix, dv, 150, subject, d, 3, offset,
+                  nh.NebulaParams(cutoff_cell=0.0), ENGINES[engine])
+    from fixtures import read_table
+    _, exp = read_table("nebula_expected.csv")
+    assert list(fit.gene_index) == list(range(8))
+    assert_close(fit.coefficients, exp[:, 0:3], *GPU_TOLS["coef"], label="coef")
+    assert_close(fit.se, exp[:, 3:6], *GPU_TOLS["se"], label="se")
+    assert_close(fit.subject_overdispersion, exp[:, 10], *GPU_TOLS["subject"], label="subject")
+    assert_close(fit.cell_overdispersion, exp[:, 11], *GPU_TOLS["cell"], label="cell")
+    assert list(np.nonzero(fit.sigma_at_bound)[0]) == [1, 3, 6]
+    assert_objective_parity(ip, ix, dv, 150, subject, d, 3, offset,
+                            oracle.params(cutoff_cell=0.0), fit)
+
+
+@pytest.mark.parametrize("engine", list(ENGINES))
+@pytest.mark.parametrize("tag", [t for t in SC_DATASETS if SC_DATASETS[t][2] <= 8])
+def test_sc_dataset_matches_r(tag, engine):
+    """R nebula's values on the e2e datasets under the GPU path's gates
+    (tests/e2e_nebula.rs:1010-1074, per-dataset loosening :232-254); for HL
+    datasets also the stage-two objective parity against the oracle."""
+    from test_oracle_nebula import GPU_TOLS, check_sc
+    d = load_sc(tag)
+    ip, ix, dv = dense_to_csr(d["counts"])
+    n_cells = d["counts"].shape[1]
+    fit = run_hip(ip, ix, dv, n_cells, d["subject"], d["design"], d["n_coef"],
+                  d["offset"], nh.NebulaParams(method=d["method"]), ENGINES[engine])
+    check_sc(tag, dict(coefficients=fit.coefficients, se=fit.se, covariance=fit.covariance,
+                       subject_overdispersion=fit.subject_overdispersion,
+                       cell_overdispersion=fit.cell_overdispersion, convergence=fit.convergence,
+                       gene_index=fit.gene_index), base=GPU_TOLS)
+    if d["method"] == "HL":
+        assert_objective_parity(ip, ix, dv, n_cells, d["subject"], d["design"], d["n_coef"],
+                                d["offset"], oracle.params(method="HL"), fit)
+
+
+@pytest.mark.parametrize("engine", list(ENGINES))
+def test_synthetic_config2_shape_matches_oracle(engine):
+    """A scaled config-2 workload (Dirichlet subject sizes, p = 4): stage-two
+    minimisers re-evaluated by the oracle's f64 profile objective agree to
+    |dloglik| < 1e-6 relative; coefficients and variances to search accuracy."""
+    w = synth.config(2, seed=21, n_genes=24, n_cells=20000)
+    prm = oracle.params(method="HL")
+    ref = oracle.nebula_sparse(w.indptr, w.indices, w.data, w.n_cells, w.subject_id, w.design,
+                               w.n_coef, w.offset, prm, with_stage_two=True)
+    fit = run_hip(w.indptr, w.indices, w.data, w.n_cells, w.subject_id, w.design, w.n_coef,
+                  w.offset, nh.NebulaParams(method="HL"), ENGINES[engine])
+    assert np.array_equal(fit.gene_index, ref["gene_index"])
+    g = np.arange(len(fit.gene_index))
+    pts_hip = np.c_[fit.subject_overdispersion, 1.0 / fit.cell_overdispersion]
+    pts_cpu = ref["stage_two"][:, 1:3]
+    v_hip = oracle.profile_objective(w.indptr, w.indices, w.data, w.n_cells, w.subject_id,
+                                     w.design, w.n_coef, w.offset, prm, g, pts_hip)
+    v_cpu = oracle.profile_objective(w.indptr, w.indices, w.data, w.n_cells, w.subject_id,
+                                     w.design, w.n_coef, w.offset, prm, g, pts_cpu)
+    rel = np.abs(v_hip - v_cpu) / np.abs(v_cpu)
+    assert rel.max() < 1e-6, rel.max()
+    assert_close(fit.coefficients, ref["coefficients"], 1e-3, 1e-6, label="coef")
+    assert_close(fit.subject_overdispersion, ref["subject_overdispersion"], 2e-2, 1e-4,
+                 label="subject")
+    assert_close(fit.cell_overdispersion, ref["cell_overdispersion"], 1e-2, label="cell")

@@ -126,18 +126,33 @@ def test_error_codes():
     assert e.value.code == 10
 
 
-# Per-dataset gates (tests/e2e_nebula.rs:66-254): coef, se, cov, subject, cell.
+# Gates as (relative, absolute floor): the CPU path's (tests/e2e_nebula.rs:66-153)
+# and the GPU path's (GPU_TOLS, tests/e2e_nebula.rs:1010-1020).
 BASE = dict(coef=(3e-5, 1e-9), se=(1e-4, 0.0), cov=(1e-3, 1e-12), subject=(2e-4, 1e-6),
             cell=(1e-4, 0.0))
-LOOSER = {"sc_high": dict(coef=(3e-4, 1e-9), subject=(2e-3, 1e-6), cov=(2.5e-2, 1e-12))}
+GPU_TOLS = dict(coef=(1e-2, 1e-9), se=(1e-2, 0.0), cov=(1.5e-2, 1e-6), subject=(1.5e-2, 1e-3),
+                cell=(1e-2, 0.0))
 
 
-def check_sc(tag, fit):
+def tols(tag, base):
+    """Per-dataset loosening on top of a path's gates (tests/e2e_nebula.rs:232-254):
+    relative gates are raised to at least the dataset's figure, floors kept."""
+    t = dict(base)
+    raise_to = {"sc_blocks": dict(coef=1e-4), "sc_cat_ln": dict(coef=1e-3, subject=1e-3),
+                "sc_high": dict(coef=3e-4, subject=2e-3, cov=2.5e-2)}.get(tag, {})
+    for key, rel in raise_to.items():
+        t[key] = (max(t[key][0], rel), t[key][1])
+    if tag in ("sc_blocks", "sc_cat_ln"):
+        t["cell"] = (t["cell"][0], 3e-3)
+    return t
+
+
+def check_sc(tag, fit, base=BASE):
     d = load_sc(tag)
     nh, neb = d["expected"]
     col = {h: i for i, h in enumerate(nh)}
     n_coef = d["n_coef"]
-    tol = dict(BASE, **LOOSER.get(tag, {}))
+    tol = tols(tag, base)
     assert list(fit["gene_index"]) == list(neb[:, col["gene_id"]].astype(int) - 1)
     names = [h[len("logFC_"):] for h in nh if h.startswith("logFC_")]
     want_coef = np.column_stack([neb[:, col["logFC_" + n]] for n in names])
