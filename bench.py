@@ -46,6 +46,21 @@ def read_traffic(kernel, workload):
     return None
 
 
+def copy_calibrate(mib=256, reps=4):
+    """A plain copy of known size: each launch reads and writes mib MiB once,
+    so rocprofv3's FETCH_SIZE / WRITE_SIZE per dispatch for it, divided by
+    mib * 1024 KiB, is the counter's scale on this device."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    assert torch.equal(dst[-8:], src[-8:])
+    log(f"copy: {reps} x {mib} MiB device-to-device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,7 +75,13 @@ def main():
                     help="genes whose minimisers are re-evaluated by the f64 oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--copy-calibrate", action="store_true",
+                    help="run only a 256 MiB device-to-device copy (the scale for the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     a = ap.parse_args()
+    if a.copy_calibrate:
+        copy_calibrate()
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: stage two shards by gene across GPUs with no "
               "exchange during the search, and this build measures one GPU's share. "

@@ -83,6 +83,10 @@ int nh_resident_upload(size_t n_cells, size_t n_coef, size_t n_subjects, const d
   try {
     h->r.upload(n_cells, n_coef, n_subjects, design, log_offset, subject_start, n_genes, gene_ptr,
                 cells, counts, device);
+  } catch (const std::invalid_argument& e) {
+    delete h;
+    put_err(err, err_len, e.what());
+    return NH_ERR_INVALID_ARGUMENT;
   } catch (const std::exception& e) {
     delete h;
     put_err(err, err_len, e.what());

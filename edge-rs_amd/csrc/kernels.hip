@@ -15,24 +15,25 @@ namespace nh {
 // (edge-rs ResidentBatch::solve, src/gpu/nebula_gpu.rs:441, one fit per plane
 // there; one per workgroup here).
 template <int NB, int WAVES>
-__global__ __launch_bounds__(WAVES * 64) void pml_batch_kernel(
+__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4))) void pml_batch_kernel(
     DeviceData dd, const DevRequest* req, const double* beta_init, const double* lw_init,
     DevReply* out, double* out_beta, double* out_lw, double* out_info, double* scratch,
-    size_t stride, FitKnobs kn, int mode, FitCounters* ctr) {
+    size_t stride, FitKnobs kn, int mode, FitCounters* ctr, int work_in_lds) {
   __shared__ FitLds<NB, WAVES> sh;
+  extern __shared__ double nh_batch_dyn[];
   const int q = blockIdx.x;
   const int k = (int)dd.k;
-  FitScratch<NB> sc;
-  sc.carve(scratch + (size_t)q * stride, k, WAVES);
+  FitWork<NB> wk;
+  wk.carve(work_in_lds ? nh_batch_dyn : scratch + (size_t)q * stride, k, WAVES);
   const DevRequest r = req[q];
   if (threadIdx.x < NB) sh.beta[threadIdx.x] = beta_init[(size_t)q * NB + threadIdx.x];
   for (int s = threadIdx.x; s < k; s += WAVES * 64)
-    sc.lw[s] = lw_init ? lw_init[(size_t)q * k + s] : 0.0;
+    wk.lw[s] = lw_init ? lw_init[(size_t)q * k + s] : 0.0;
   __syncthreads();
   const double exps = exp(r.subject);
   const double alpha = 1.0 / (exps - 1.0);
   const double lambda = 1.0 / (sqrt(exps) * (exps - 1.0));
-  Fitter<NB, WAVES> f(dd, sh, sc, ctr, r.gene, r.cell, alpha, lambda);
+  Fitter<NB, WAVES> f(dd, sh, wk, ctr, r.gene, r.cell, alpha, lambda);
   FitResult res = f.run(mode, r.ord, kn);
   if (threadIdx.x == 0) {
     DevReply o;
@@ -48,27 +49,29 @@ __global__ __launch_bounds__(WAVES * 64) void pml_batch_kernel(
   }
   if (threadIdx.x < NB) out_beta[(size_t)q * NB + threadIdx.x] = sh.beta[threadIdx.x];
   if (threadIdx.x < NB * NB) out_info[(size_t)q * NB * NB + threadIdx.x] = sh.vb2[threadIdx.x];
-  for (int s = threadIdx.x; s < k; s += WAVES * 64) out_lw[(size_t)q * k + s] = sc.lw[s];
+  for (int s = threadIdx.x; s < k; s += WAVES * 64) out_lw[(size_t)q * k + s] = wk.lw[s];
 }
 
-size_t fit_scratch_doubles(int nb, int k, int waves) {
+size_t fit_work_doubles(int nb, int k, int waves) {
   switch (nb) {
-    case 1: return FitScratch<1>::doubles(k, waves);
-    case 2: return FitScratch<2>::doubles(k, waves);
-    case 3: return FitScratch<3>::doubles(k, waves);
-    case 4: return FitScratch<4>::doubles(k, waves);
-    case 5: return FitScratch<5>::doubles(k, waves);
-    case 6: return FitScratch<6>::doubles(k, waves);
-    case 7: return FitScratch<7>::doubles(k, waves);
-    default: return FitScratch<8>::doubles(k, waves);
+    case 1: return FitWork<1>::doubles(k, waves);
+    case 2: return FitWork<2>::doubles(k, waves);
+    case 3: return FitWork<3>::doubles(k, waves);
+    case 4: return FitWork<4>::doubles(k, waves);
+    case 5: return FitWork<5>::doubles(k, waves);
+    case 6: return FitWork<6>::doubles(k, waves);
+    case 7: return FitWork<7>::doubles(k, waves);
+    default: return FitWork<8>::doubles(k, waves);
   }
 }
 
 template <int NB, int WAVES>
 static hipError_t launch_batch_t(const BatchLaunch& L, hipStream_t st) {
-  hipLaunchKernelGGL((pml_batch_kernel<NB, WAVES>), dim3(L.n_req), dim3(WAVES * 64), 0, st, L.dd,
-                     L.req, L.beta_init, L.lw_init, L.out, L.out_beta, L.out_lw, L.out_info,
-                     L.scratch, L.stride, L.knobs, L.mode, L.counters);
+  const size_t lds = L.work_in_lds ? L.stride * sizeof(double) : 0;
+  hipLaunchKernelGGL((pml_batch_kernel<NB, WAVES>), dim3(L.n_req), dim3(WAVES * 64), lds, st,
+                     L.dd, L.req, L.beta_init, L.lw_init, L.out, L.out_beta, L.out_lw,
+                     L.out_info, L.scratch, L.stride, L.knobs, L.mode, L.counters,
+                     L.work_in_lds);
   return hipGetLastError();
 }
 

@@ -30,8 +30,9 @@ struct BatchLaunch {
   double* out_beta;
   double* out_lw;
   double* out_info;
-  double* scratch;
-  size_t stride;  // doubles per request
+  double* scratch;     // global work area when it does not go in LDS
+  size_t stride;       // work doubles per request
+  int work_in_lds;     // 1: work arrays in dynamic LDS
   FitKnobs knobs;
   int mode;       // 0 f32 + f64 finish, 1 f64
   int waves;      // 4, 8 or 16
@@ -39,7 +40,10 @@ struct BatchLaunch {
   FitCounters* counters;
 };
 
-size_t fit_scratch_doubles(int nb, int k, int waves);
+size_t fit_work_doubles(int nb, int k, int waves);
+// Largest per-workgroup work area placed in LDS (bytes); larger ones live in
+// global memory.
+constexpr size_t kWorkLdsBytes = 48 * 1024;
 hipError_t launch_batch(const BatchLaunch& L, hipStream_t st);
 hipError_t launch_bobyqa_trace(int problem, int n, const double* x0, const double* lo,
                                const double* hi, double* trace, int max_trace, int* result,

@@ -48,8 +48,16 @@ __device__ static double device_objective(double subject, double cell, const Fit
   return isfinite(v) ? -v : INFINITY;
 }
 
+// The stepper runs on one lane between fits; kept out of line so the sweep
+// loop's register allocation is not shaped by it.
+__device__ __noinline__ static void bq_init(Bobyqa<2>* b, const double* x0, const double* lo,
+                                            const double* hi) {
+  b->init(2, x0, lo, hi);
+}
+__device__ __noinline__ static void bq_tell(Bobyqa<2>* b, double v) { b->tell(v); }
+
 template <int NB, int WAVES>
-__global__ __launch_bounds__(WAVES * 64) void stage_two_kernel(PersistentArgs A) {
+__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4))) void stage_two_kernel(PersistentArgs A) {
   constexpr int NT = WAVES * 64;
   __shared__ FitLds<NB, WAVES> sh;
   __shared__ Bobyqa<2> bq;
@@ -57,13 +65,14 @@ __global__ __launch_bounds__(WAVES * 64) void stage_two_kernel(PersistentArgs A)
   __shared__ double red[NT / 64];
   __shared__ int s_slot, s_active, s_found, s_failed, s_have_warm, s_evals, s_order, s_ok;
   __shared__ double s_subject, s_cell, s_value;
+  extern __shared__ double nh_s2_dyn[];
   const int tid = threadIdx.x;
   const DeviceData& dd = A.dd;
   const int k = (int)dd.k;
-  FitScratch<NB> sc;
-  double* base = A.scratch + (size_t)blockIdx.x * A.stride;
-  sc.carve(base, k, WAVES);
-  double* warm_lw = base + FitScratch<NB>::doubles(k, WAVES);
+  FitWork<NB> wk;
+  double* base = A.work_in_lds ? nh_s2_dyn : A.scratch + (size_t)blockIdx.x * A.stride;
+  wk.carve(base, k, WAVES);
+  double* warm_lw = base + FitWork<NB>::doubles(k, WAVES);
   for (;;) {
     if (tid == 0) s_slot = (int)atomicAdd(A.queue, 1u);
     __syncthreads();
@@ -79,7 +88,7 @@ __global__ __launch_bounds__(WAVES * 64) void stage_two_kernel(PersistentArgs A)
       s_failed = 0;
       const double lo[2] = {A.lo[0], A.lo[1]}, hi[2] = {A.hi[0], A.hi[1]};
       double x0[2] = {fmin(fmax(S.start[0], lo[0]), hi[0]), fmin(fmax(S.start[1], lo[1]), hi[1])};
-      bq.init(2, x0, lo, hi);
+      bq_init(&bq, x0, lo, hi);
       s_active = bq.status != BOBYQA_BAD_BOX && bq.want;
     }
     __syncthreads();
@@ -96,17 +105,17 @@ __global__ __launch_bounds__(WAVES * 64) void stage_two_kernel(PersistentArgs A)
         if (tid == (int)A.intercept) b -= subject / 2.0;
         sh.beta[tid] = s_have_warm ? warm_beta[tid] : b;
       }
-      for (int s = tid; s < k; s += NT) sc.lw[s] = s_have_warm ? warm_lw[s] : 0.0;
+      for (int s = tid; s < k; s += NT) wk.lw[s] = s_have_warm ? warm_lw[s] : 0.0;
       __syncthreads();
       const double exps = exp(subject);
       const double alpha = 1.0 / (exps - 1.0);
       const double lambda = 1.0 / (sqrt(exps) * (exps - 1.0));
-      Fitter<NB, WAVES> f(dd, sh, sc, A.counters, S.gene, cell, alpha, lambda);
+      Fitter<NB, WAVES> f(dd, sh, wk, A.counters, S.gene, cell, alpha, lambda);
       const FitResult r = f.run(0, (unsigned)s_order, A.knobs);
       // finite point and the lgamma tail over the count histogram
       double part = 0.0;
       bool fin = true;
-      for (int s = tid; s < k; s += NT) fin = fin && isfinite(sc.lw[s]);
+      for (int s = tid; s < k; s += NT) fin = fin && isfinite(wk.lw[s]);
       if (tid < NB) fin = fin && isfinite(sh.beta[tid]);
       for (uint32_t i = (uint32_t)tid; i < S.tail_len; i += NT)
         part += A.tail_mult[S.tail_off + i] * lgamma(A.tail_val[S.tail_off + i] + cell);
@@ -132,7 +141,7 @@ __global__ __launch_bounds__(WAVES * 64) void stage_two_kernel(PersistentArgs A)
       __syncthreads();
       if (s_ok) {
         if (tid < NB) warm_beta[tid] = sh.beta[tid];
-        for (int s = tid; s < k; s += NT) warm_lw[s] = sc.lw[s];
+        for (int s = tid; s < k; s += NT) warm_lw[s] = wk.lw[s];
       }
       __syncthreads();
       if (tid == 0) {
@@ -142,7 +151,7 @@ __global__ __launch_bounds__(WAVES * 64) void stage_two_kernel(PersistentArgs A)
         if (!isfinite(v)) {
           pass_over = true;
         } else {
-          bq.tell(v);
+          bq_tell(&bq, v);
           if (!bq.want) {
             pass_over = true;
             found = true;
@@ -158,7 +167,7 @@ __global__ __launch_bounds__(WAVES * 64) void stage_two_kernel(PersistentArgs A)
             const double lo[2] = {A.lo[0], A.lo[1]}, hi[2] = {A.hi[0], A.hi[1]};
             double x0[2] = {fmin(fmax(S.start[0], lo[0]), hi[0]),
                             fmin(fmax(S.start[1], lo[1]), hi[1])};
-            bq.init(2, x0, lo, hi);
+            bq_init(&bq, x0, lo, hi);
             s_active = bq.status != BOBYQA_BAD_BOX && bq.want;
           }
         }
@@ -184,12 +193,13 @@ __global__ void reset_queue_kernel(unsigned* q) {
   if (threadIdx.x == 0) *q = 0u;
 }
 
-size_t persistent_stride(int nb, int k, int waves) { return fit_scratch_doubles(nb, k, waves) + (size_t)k; }
+size_t persistent_stride(int nb, int k, int waves) { return fit_work_doubles(nb, k, waves) + (size_t)k; }
 
 template <int NB, int WAVES>
 static hipError_t launch_t(const PersistentArgs& A, int grid, hipStream_t st) {
   hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, st, A.queue);
-  hipLaunchKernelGGL((stage_two_kernel<NB, WAVES>), dim3(grid), dim3(WAVES * 64), 0, st, A);
+  const size_t lds = A.work_in_lds ? A.stride * sizeof(double) : 0;
+  hipLaunchKernelGGL((stage_two_kernel<NB, WAVES>), dim3(grid), dim3(WAVES * 64), lds, st, A);
   return hipGetLastError();
 }
 
@@ -282,7 +292,8 @@ void run_stage_two_batched(Prepared& P, Resident& R, const nebula_hip_options_t&
   dqueue.alloc(1);
   dout.alloc(n);
   const size_t stride = persistent_stride((int)nb, (int)k, waves);
-  dscr.alloc(stride * (size_t)grid);
+  const bool in_lds = stride * sizeof(double) <= kWorkLdsBytes;
+  if (!in_lds) dscr.alloc(stride * (size_t)grid);
   PersistentArgs A;
   A.dd = R.view();
   A.search = dsearch.p;
@@ -292,8 +303,9 @@ void run_stage_two_batched(Prepared& P, Resident& R, const nebula_hip_options_t&
   A.tail_val = dtv.p;
   A.tail_mult = dtm.p;
   A.out = dout.p;
-  A.scratch = dscr.p;
+  A.scratch = in_lds ? nullptr : dscr.p;
   A.stride = stride;
+  A.work_in_lds = in_lds ? 1 : 0;
   A.knobs.eps = p.eps;
   A.knobs.noise_rel = 1e-8;
   A.knobs.max_iter = 50;

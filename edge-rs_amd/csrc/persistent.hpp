@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "kernels.hpp"
 #include "pml_fit.hpp"
 
 namespace nh {
@@ -33,8 +34,9 @@ struct PersistentArgs {
   const double* tail_val;
   const double* tail_mult;
   DevSearchOut* out;
-  double* scratch;
-  size_t stride;          // doubles per workgroup
+  double* scratch;        // global work area when it does not go in LDS
+  size_t stride;          // work doubles per workgroup
+  int work_in_lds;        // 1: work arrays in dynamic LDS
   FitKnobs knobs;
   double lo[2], hi[2];
   uint32_t intercept;

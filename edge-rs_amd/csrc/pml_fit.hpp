@@ -1,31 +1,45 @@
 // One penalised NB fit (edge-rs opt_pml, src/sc/pml.rs:1614-1800) per HIP
 // workgroup, for gfx950.
 //
+// Data layout.  Every cell has one packed row in HBM: the design columns other
+// than the intercept, then the log offset, zero-padded to R = 1, 2, 4 or 8
+// values (16 bytes per cell in f32 at p = 4, one dwordx4 load).  The intercept
+// column is 1 for every cell (nebula's centred design), so its sums are copies
+// of sums the fit keeps anyway: score[int] = sum(resid), first[int] = weight,
+// second[int][j] = first[j], second[int][int] = weight.  A fit therefore keeps
+// 2 + Q per-subject and 1 + Q + Q(Q+1)/2 per-gene accumulators (Q = p - 1)
+// instead of 3 + 2p + p(p+1)/2.
+//
 // Work split.  The workgroup's W waves cut the gene's cells into W contiguous
-// blocks (64-cell aligned).  A wave walks its block subject by subject in rows
-// of 64 consecutive cells, so design rows and log offsets arrive as coalesced
-// loads.  The gene's positive counts are merged in on the fly: the wave keeps
-// a cursor into the gene's sorted CSR row, ballots the entries that fall in the
-// current row and scatters their counts into a 64-slot LDS row, so every cell
-// is visited once with its own count.  That one pass is edge-rs's dense sweep
-// at y = 0 plus its sparse correction (pml.rs:1360-1363, exact because every
-// per-cell term is linear in y), without the second visit to positive cells.
+// blocks (64-cell aligned).  A wave walks its block subject by subject in tiles
+// of 64 x U consecutive cells (U = 4 in f32, 2 in f64): each lane loads U
+// packed rows, all independent, so a tile keeps U loads per lane in flight.
+// The gene's positive counts are merged in with no dependent load: the wave
+// holds a window of the next 64 x U entries of the gene's sorted CSR row in
+// registers (a tile of 64 x U cells holds at most that many), scatters the
+// entries that fall in the tile into a per-wave LDS row, advances its cursor
+// by the ballot count and loads the next window before it computes the tile.
+// Every cell is visited once with its own count; that one pass is edge-rs's
+// dense sweep at y = 0 plus its sparse correction (pml.rs:1360-1363, exact
+// because every per-cell term is linear in y).
 //
 // Precision.  Per cell the exp, log and reciprocal run in CT: float for the
 // bulk Newton sweeps, double for the finishing sweeps.  Every accumulation is
-// FP64 (on CDNA4 an FP64 FMA issues at the FP32 rate), so the long sums over
-// 1e5..1e6 cells carry no summation error and the Schur complement
-// vb - sum vwb vwb'/vw is formed from exact sums.  What FP32 leaves is the
-// per-cell rounding of exp/log, which the FP64 finish removes: one f64 Newton
-// step from the f32 optimum and the log-likelihood and log-determinant at the
-// stepped point (edge-rs newton_finish, src/sc/pml.rs:807-1090).  If that step
-// does not settle the fit, the whole Newton loop reruns in f64 from the f32
+// FP64 (on CDNA4 a DFMA issues at the rate of a non-packed FP32 FMA), so the
+// long sums over 1e5..1e6 cells carry no summation error and the Schur
+// complement vb - sum vwb vwb'/vw is formed from exact sums.  What FP32 leaves
+// is the per-cell rounding of exp/log, which the FP64 finish removes: f64
+// Newton steps from the f32 optimum and the log-likelihood and log-determinant
+// at the stepped point (edge-rs newton_finish, src/sc/pml.rs:807-1090).  If
+// the finish does not settle, the whole Newton loop reruns in f64 from the f32
 // point with opt_pml's exact control flow (opt_pml_from, pml.rs:637).
 //
-// Reductions.  Per subject, each wave reduces its lane partials with a fixed
-// butterfly and lane 0 stores them; the workgroup then adds the (at most two
-// or three) wave partials of a subject in wave order.  Results do not depend
-// on timing.
+// Reductions.  Lane partials are reduced with a fixed butterfly, wave partials
+// are added in wave order and per-subject sums over subjects are taken by one
+// wave per quantity with a fixed lane assignment, so results do not depend on
+// timing.  Everything between sweeps (subject sums, Schur complement, the
+// pivoted LDL' solve, damping) works on LDS, or on the global work area when
+// the subject count makes the LDS one too large.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -37,17 +51,17 @@
 namespace nh {
 
 struct DeviceData {
-  const float* x32;      // n_cells * p4, centred design, zero-padded columns
-  const double* x64;     // n_cells * nb
-  const float* off32;    // log offset
-  const double* off64;
+  const float* row32;    // n_cells * rw: non-intercept design columns, log offset, padding
+  const double* row64;   // the same in f64
+  const double* x64;     // n_cells * nb centred design (stage one)
+  const double* off64;   // log offset (stage one)
   const uint32_t* fid;   // k + 1 subject boundaries
   const uint64_t* gptr;  // n_genes + 1 CSR row starts
   const uint32_t* gcell; // positive cells, increasing within a gene
   const float* gy32;     // positive counts
   const double* gy64;
   const double* gtot;    // n_genes * k subject totals
-  uint32_t n_cells, nb, p4, k;
+  uint32_t n_cells, nb, k, icpt, rw;
 };
 
 struct FitKnobs {
@@ -64,39 +78,67 @@ struct FitCounters {
   unsigned long long cells_f64, nnz_f64;  // summed over FP64 sweeps
 };
 
-enum SweepMode { SW_FULL = 0, SW_VALUE = 1, SW_LAPLACE = 2 };
+enum SweepMode { SW_FULL = 0, SW_LAPLACE = 2 };
+
+// Cells per lane in one tile of a sweep (tile = 64 x U cells).
+#ifndef NH_TILE_F32
+#define NH_TILE_F32 2
+#endif
+#ifndef NH_TILE_F64
+#define NH_TILE_F64 1
+#endif
+constexpr int kTileF32 = NH_TILE_F32;
+constexpr int kTileF64 = NH_TILE_F64;
 
 template <int NB>
 struct Dims {
+  static constexpr int Q = NB - 1;                // stored design columns
+  static constexpr int R = Q + 1 <= 1 ? 1 : (Q + 1 <= 2 ? 2 : (Q + 1 <= 4 ? 4 : 8));
+  static constexpr int QT = Q * (Q + 1) / 2;
+  static constexpr int NS = 2 + Q;                // per subject: resid, weight, first[Q]
+  static constexpr int NSL = NS > 4 ? NS : 4;     // slot width (Laplace keeps 4)
+  static constexpr int NG = 1 + Q + QT;           // per gene: llp, score[Q], second[QT]
+  static constexpr int SUB = 3 + NB;              // dw, vw, w, vwb[NB]
   static constexpr int NTRI = NB * (NB + 1) / 2;
-  static constexpr int V = 3 + 2 * NB + NTRI;  // resid, weight, llp, score, first, second
-  static constexpr int SUB = 3 + NB;           // dw, vw, w, vwb[NB]
 };
 
-// Per-workgroup scratch in global memory.
+__host__ __device__ inline uint32_t packed_row_width(uint32_t nb) {
+  const uint32_t r = nb;  // nb - 1 columns + offset
+  return r <= 1 ? 1u : (r <= 2 ? 2u : (r <= 4 ? 4u : 8u));
+}
+
+// Per-fit work arrays: k-sized and partial-sum buffers, in LDS when they fit.
 template <int NB>
-struct FitScratch {
-  double* part;     // (k + W) * V   wave partials, slot (s + w)
-  double* tot;      // k * V         per-subject totals
-  double* sub[2];   // k * SUB       per side: dw, vw, w, vwb
-  double* llsub;    // k
+struct FitWork {
+  using D = Dims<NB>;
+  double* psub;     // (k + W) * NSL  wave partials per subject, slot s + w
+  double* pgene;    // W * NG         wave partials per gene
+  double* tsub;     // k * NSL        per-subject totals
+  double* tgene;    // NG             per-gene totals
+  double* llsub;    // k              prior and count terms of the log-likelihood
+  double* subs;     // 2 * k * SUB    per side: dw, vw, w, vwb
   double* lw;       // k  current log w
   double* new_lw;   // k  trial
   double* step_lw;  // k
   double* damp_lw;  // k
   double* vw_step;  // k  vw of the last Newton step (log-det, pml.rs:1767)
+  __host__ __device__ static size_t doubles(int k, int waves) {
+    return (size_t)(k + waves) * D::NSL + (size_t)waves * D::NG + (size_t)k * D::NSL + D::NG +
+           (size_t)k + 2 * (size_t)k * D::SUB + 5 * (size_t)k;
+  }
   __device__ void carve(double* base, int k, int waves) {
-    constexpr int V = Dims<NB>::V;
-    part = base;
-    base += (size_t)(k + waves) * V;
-    tot = base;
-    base += (size_t)k * V;
-    sub[0] = base;
-    base += (size_t)k * Dims<NB>::SUB;
-    sub[1] = base;
-    base += (size_t)k * Dims<NB>::SUB;
+    psub = base;
+    base += (size_t)(k + waves) * D::NSL;
+    pgene = base;
+    base += (size_t)waves * D::NG;
+    tsub = base;
+    base += (size_t)k * D::NSL;
+    tgene = base;
+    base += D::NG;
     llsub = base;
     base += k;
+    subs = base;
+    base += 2 * (size_t)k * D::SUB;
     lw = base;
     base += k;
     new_lw = base;
@@ -107,10 +149,6 @@ struct FitScratch {
     base += k;
     vw_step = base;
   }
-  __host__ __device__ static size_t doubles(int k, int waves) {
-    return (size_t)(k + waves) * Dims<NB>::V + (size_t)k * Dims<NB>::V +
-           2 * (size_t)k * Dims<NB>::SUB + 6 * (size_t)k;
-  }
 };
 
 template <int NB, int WAVES>
@@ -120,12 +158,14 @@ struct FitLds {
   double vb[2][NB * NB];
   double vb2[NB * NB];
   double rhs[NB];
+  double lm[NB * NB];  // LDL' work
+  double lb[NB];
+  int perm[NB];
   double ll[2];
-  double ysc[WAVES][kWave];
+  double scal[8];
   uint64_t ip0[WAVES];
   uint32_t s0[WAVES];
-  double scal[8];
-  int flag[4];
+  float ysc[WAVES][kWave * 4];  // count staging: 256 floats or 128 doubles per wave
 };
 
 // What one fit hands back.
@@ -135,13 +175,11 @@ struct FitResult {
 };
 
 // ---------------------------------------------------------------------------
-// Eigen-style pivoted LDL' solve of a small symmetric system, in place
+// Eigen-style pivoted LDL' solve of a small symmetric system in LDS, in place
 // (src/sc/pml.rs:1940-2021).  One thread.
 template <int NB>
-__device__ void ldlt_solve_dev(double* a, double* b) {
+__device__ void ldlt_solve_lds(double* a, double* b, int* perm) {
   const int n = NB;
-  int perm[NB];
-  double tmp[NB];
   for (int st = 0; st < n; ++st) {
     int piv = st;
     double best = fabs(a[st * n + st]);
@@ -174,13 +212,12 @@ __device__ void ldlt_solve_dev(double* a, double* b) {
       }
     }
     if (st > 0) {
-      for (int j = 0; j < st; ++j) tmp[j] = a[j * n + j] * a[st * n + j];
       double acc = 0.0;
-      for (int j = 0; j < st; ++j) acc += a[st * n + j] * tmp[j];
+      for (int j = 0; j < st; ++j) acc += a[st * n + j] * (a[j * n + j] * a[st * n + j]);
       a[st * n + st] -= acc;
       for (int i = st + 1; i < n; ++i) {
         double ac = 0.0;
-        for (int j = 0; j < st; ++j) ac += a[i * n + j] * tmp[j];
+        for (int j = 0; j < st; ++j) ac += a[i * n + j] * (a[j * n + j] * a[st * n + j]);
         a[i * n + st] -= ac;
       }
     }
@@ -233,30 +270,236 @@ __device__ __forceinline__ float drcp<float>(float x) { return __frcp_rn(x); }
 template <>
 __device__ __forceinline__ double drcp<double>(double x) { return 1.0 / x; }
 
+// One packed row of R values (16-byte vector loads where the row allows).
+template <class CT, int R>
+__device__ __forceinline__ void load_row(const CT* __restrict__ base, uint32_t cell, CT* out) {
+  const CT* p = base + (size_t)cell * R;
+  if constexpr (sizeof(CT) == 4 && R >= 4) {
+#pragma unroll
+    for (int h = 0; h < R / 4; ++h) {
+      const float4 v = reinterpret_cast<const float4*>(p)[h];
+      out[4 * h] = v.x;
+      out[4 * h + 1] = v.y;
+      out[4 * h + 2] = v.z;
+      out[4 * h + 3] = v.w;
+    }
+  } else if constexpr (sizeof(CT) == 4 && R == 2) {
+    const float2 v = *reinterpret_cast<const float2*>(p);
+    out[0] = v.x;
+    out[1] = v.y;
+  } else if constexpr (sizeof(CT) == 8 && R >= 2) {
+#pragma unroll
+    for (int h = 0; h < R / 2; ++h) {
+      const double2 v = reinterpret_cast<const double2*>(p)[h];
+      out[2 * h] = v.x;
+      out[2 * h + 1] = v.y;
+    }
+  } else {
+    out[0] = p[0];
+  }
+}
+
+// index of (a, b), a <= b, in the packed upper triangle of a Q x Q matrix
+__host__ __device__ constexpr int tri_index(int q, int a, int b) {
+  return a * q - a * (a - 1) / 2 + (b - a);
+}
+
+typedef __attribute__((address_space(3))) float lds_float;
+typedef __attribute__((address_space(3))) double lds_double;
+template <class CT>
+struct Lds;
+template <>
+struct Lds<float> {
+  using ptr = lds_float*;
+};
+template <>
+struct Lds<double> {
+  using ptr = lds_double*;
+};
+
+// Raw hardware transcendentals for the FP32 sweeps: v_exp_f32 / v_log_f32
+// (base 2) and v_rcp_f32, about 1 ulp; the arguments here are normal numbers
+// (t = e + gamma >= gamma > 0).
+__device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504f); }
+__device__ __forceinline__ float fast_log(float x) { return __builtin_amdgcn_logf(x) * 0.693147181f; }
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double fast_exp(double x) { return exp(x); }
+__device__ __forceinline__ double fast_log(double x) { return log(x); }
+__device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; }
+
+// One wave's part of a sweep: cells [cb, ce), subjects from s, CSR cursor ip
+// (first entry at or past cb).  Writes the wave's per-subject sums to psub
+// (slot s + wave) and its per-gene sums to pgene (row wave).  Everything it
+// reads comes in as restrict-qualified arguments so loads can be hoisted and
+// kept in flight; the count staging row is addressed as LDS.
+template <class CT, int MODE, int NB>
+__device__ __forceinline__ void sweep_wave(const CT* __restrict__ rows,
+                                        const uint32_t* __restrict__ gcell,
+                                        const CT* __restrict__ gy,
+                                        const uint32_t* __restrict__ fid,
+                                        const double* __restrict__ bsrc,
+                                        const double* __restrict__ lwsrc,
+                                        typename Lds<CT>::ptr ys, double* __restrict__ psub,
+                                        double* __restrict__ pgene, uint64_t ip, uint64_t g1,
+                                        int s, int k, uint32_t cb, uint32_t ce, int icpt,
+                                        double gamma, int lane, int wave) {
+  using D = Dims<NB>;
+  constexpr int Q = D::Q, R = D::R, NS = D::NS, NSL = D::NSL, NG = D::NG;
+  constexpr int U = sizeof(CT) == 4 ? kTileF32 : kTileF64;
+  constexpr int TILE = kWave * U;
+  constexpr int NSV = MODE == SW_LAPLACE ? 4 : NS;
+  constexpr int NGV = MODE == SW_LAPLACE ? 0 : NG;
+  constexpr int QA = Q > 0 ? Q : 1;
+  CT bq[QA];
+#pragma unroll
+  for (int q = 0; q < QA; ++q) bq[q] = q < Q ? (CT)bsrc[q < icpt ? q : q + 1] : (CT)0;
+  const CT bint = (CT)bsrc[icpt];
+  const CT g = (CT)gamma;
+  double ag[NGV > 0 ? NGV : 1];
+#pragma unroll
+  for (int v = 0; v < (NGV > 0 ? NGV : 1); ++v) ag[v] = 0.0;
+  if (cb < ce) {
+    // window of the next TILE entries of the gene's CSR row
+    uint32_t wc[U];
+    CT wy[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const uint64_t e = ip + (uint64_t)(lane + kWave * j);
+      const bool ok = e < g1;
+      wc[j] = ok ? gcell[e] : 0xffffffffu;
+      wy[j] = ok ? gy[e] : (CT)0;
+    }
+    for (; s < k && fid[s] < ce; ++s) {
+      const uint32_t lo = max(fid[s], cb), hi = min(fid[s + 1], ce);
+      const CT lws = (CT)lwsrc[s];
+      const CT c0 = bint + lws;
+      double as[NSV];
+#pragma unroll
+      for (int v = 0; v < NSV; ++v) as[v] = 0.0;
+      for (uint32_t base = lo; base < hi; base += TILE) {
+        const uint32_t tend = min(base + (uint32_t)TILE, hi);
+        CT xr[U][R];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint32_t cell = base + (uint32_t)(lane + kWave * u);
+          if (cell < tend) {
+            load_row<CT, R>(rows, cell, xr[u]);
+          } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) xr[u][r] = (CT)0;
+          }
+        }
+        // the window's entries inside [base, tend) form a prefix of it
+        uint64_t m = 0;
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+          const bool in = wc[j] < tend;
+          if (in) ys[wc[j] - base] = wy[j];
+          m += (uint64_t)popc64(wave_ballot(in));
+        }
+        ip += m;
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+          const uint64_t e = ip + (uint64_t)(lane + kWave * j);
+          const bool ok = e < g1;
+          wc[j] = ok ? gcell[e] : 0xffffffffu;
+          wy[j] = ok ? gy[e] : (CT)0;
+        }
+        __builtin_amdgcn_wave_barrier();
+        CT yv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) yv[u] = ys[lane + kWave * u];
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int u = 0; u < U; ++u) ys[lane + kWave * u] = (CT)0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint32_t cell = base + (uint32_t)(lane + kWave * u);
+          if (cell >= tend) continue;
+          const CT y = yv[u];
+          CT eta = xr[u][Q];  // log offset
+#pragma unroll
+          for (int q = 0; q < Q; ++q) eta += xr[u][q] * bq[q];
+          const CT ex = fast_exp(eta + c0);
+          eta += bint;
+          const CT t = ex + g;
+          const CT inv = fast_rcp(t);
+          if constexpr (MODE == SW_LAPLACE) {
+            // (gamma + y) times the four derivative factors (pml.rs:1861-1874)
+            double e64 = (double)ex, t64 = e64 + gamma, gs64 = gamma + (double)y;
+            double p2 = e64 / (t64 * t64), p3 = p2 / t64, p4 = p3 / t64, p5 = p4 / t64;
+            double e2 = e64 * e64;
+            as[0] += gs64 * p2;
+            as[1] += gs64 * (p3 * (gamma - e64));
+            as[2] += gs64 * (p4 * (gamma * gamma + e2 - (4.0 * gamma) * e64));
+            as[3] += gs64 * (p5 * (gamma * gamma * gamma - (11.0 * gamma * gamma) * e64 +
+                                   (11.0 * gamma) * e2 - e2 * e64));
+          } else {
+            const CT u_ = ex * inv;
+            const CT gs = g + y;
+            const CT lt = fast_log(t);
+            const double pw = (double)(gs * u_ * inv);
+            const double dres = (double)y - (double)gs * (double)u_;
+            const double term = (double)(y * eta) - (double)gs * (double)lt;
+            as[0] += dres;
+            as[1] += pw;
+            ag[0] += term;
+            double xd[QA];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) xd[q] = (double)xr[u][q];
+#pragma unroll
+            for (int a = 0; a < Q; ++a) {
+              const double pa = pw * xd[a];
+              as[2 + a] += pa;
+              ag[1 + a] += xd[a] * dres;
+#pragma unroll
+              for (int b = a; b < Q; ++b) ag[1 + Q + tri_index(Q, a, b)] += pa * xd[b];
+            }
+          }
+        }
+      }
+      // flush subject s: wave sum, slot s + wave
+#pragma unroll
+      for (int v = 0; v < NSV; ++v) {
+        const double r = wave_sum(as[v]);
+        if (lane == 0) psub[(size_t)(s + wave) * NSL + v] = r;
+      }
+    }
+  }
+  if constexpr (NGV > 0) {
+#pragma unroll
+    for (int v = 0; v < NGV; ++v) {
+      const double r = wave_sum(ag[v]);
+      if (lane == 0) pgene[(size_t)wave * NG + v] = r;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // The fit of one request, all threads of the workgroup cooperating.
 template <int NB, int WAVES>
 struct Fitter {
+  using D = Dims<NB>;
   static constexpr int NT = WAVES * kWave;
-  static constexpr int V = Dims<NB>::V;
-  static constexpr int SUB = Dims<NB>::SUB;
-  static constexpr int NTRI = Dims<NB>::NTRI;
+  static constexpr int Q = D::Q, R = D::R, QT = D::QT, NS = D::NS, NSL = D::NSL, NG = D::NG;
+  static constexpr int SUB = D::SUB, NTRI = D::NTRI;
 
   const DeviceData& dd;
   FitLds<NB, WAVES>& sh;
-  FitScratch<NB>& sc;
+  FitWork<NB>& wk;
   FitCounters* counters;
   uint32_t gene;
   uint64_t g0, g1;   // CSR range
   double gamma, alpha, lambda;
-  int k;
+  int k, icpt;
   uint32_t blk;      // cells per wave block
   int tid, wave, lane;
 
-  __device__ Fitter(const DeviceData& d, FitLds<NB, WAVES>& s, FitScratch<NB>& c,
+  __device__ Fitter(const DeviceData& d, FitLds<NB, WAVES>& s, FitWork<NB>& w,
                     FitCounters* ctr, uint32_t g, double gam, double alp, double lam)
-      : dd(d), sh(s), sc(c), counters(ctr), gene(g), gamma(gam), alpha(alp), lambda(lam) {
+      : dd(d), sh(s), wk(w), counters(ctr), gene(g), gamma(gam), alpha(alp), lambda(lam) {
     k = (int)dd.k;
+    icpt = (int)dd.icpt;
     g0 = dd.gptr[g];
     g1 = dd.gptr[g + 1];
     uint32_t per = (dd.n_cells + WAVES - 1) / WAVES;
@@ -266,12 +509,18 @@ struct Fitter {
     lane = tid % kWave;
   }
 
+  __device__ double* side(int sd) const { return wk.subs + (size_t)sd * k * SUB; }
+
+  // design column of stored column q
+  __device__ static constexpr int qcol(int q, int ic) { return q < ic ? q : q + 1; }
+
   __device__ uint32_t block_begin(int w) const { return min(dd.n_cells, (uint32_t)w * blk); }
   __device__ uint32_t block_end(int w) const { return min(dd.n_cells, (uint32_t)(w + 1) * blk); }
 
   // Per wave: the first positive entry at or past the block start (64-ary
-  // search over the gene's sorted cells) and the subject holding it.
-  __device__ void setup() {
+  // search over the gene's sorted cells) and the subject holding it; clears
+  // the wave's count staging row.
+  __device__ __noinline__ void setup() {
     uint32_t cb = block_begin(wave);
     uint64_t lo = g0, hi = g1;
     while (hi - lo > (uint64_t)kWave) {
@@ -291,132 +540,48 @@ struct Fitter {
     uint64_t pos = lo + (uint64_t)lane;
     bool pred = pos < hi && dd.gcell[pos] < cb;
     uint64_t ip = lo + (uint64_t)popc64(wave_ballot(pred));
-    // subject containing cb: last s with fid[s] <= cb
-    int s = 0;
-    {
-      int a = 0, b = k;  // fid[a] <= cb < fid[b] (when cb < n_cells)
-      while (b - a > 1) {
-        int m = (a + b) / 2;
-        if (dd.fid[m] <= cb) a = m; else b = m;
-      }
-      s = a;
+    int a = 0, b = k;  // fid[a] <= cb < fid[b] (when cb < n_cells)
+    while (b - a > 1) {
+      int m = (a + b) / 2;
+      if (dd.fid[m] <= cb) a = m; else b = m;
     }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) sh.ysc[wave][lane + kWave * u] = 0.0f;
     if (lane == 0) {
       sh.ip0[wave] = ip;
-      sh.s0[wave] = (uint32_t)s;
+      sh.s0[wave] = (uint32_t)a;
     }
     __syncthreads();
   }
 
-  // One pass over the gene's cells at (beta in LDS `bsrc`, log w in global
-  // `lwsrc`), leaving per-subject totals in sc.tot.
+  // One pass over the gene's cells at (beta in LDS `bsrc`, log w `lwsrc`),
+  // leaving per-subject totals in wk.tsub and per-gene totals in wk.tgene.
   template <class CT, int MODE>
-  __device__ void sweep(const double* bsrc, const double* lwsrc) {
-    constexpr int NV = MODE == SW_FULL ? V : (MODE == SW_VALUE ? 2 : 4);
-    CT b[NB];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) b[j] = (CT)bsrc[j];
-    const CT g = (CT)gamma;
-    const uint32_t cb = block_begin(wave), ce = block_end(wave);
-    uint64_t ip = sh.ip0[wave];
-    int s = (int)sh.s0[wave];
-    if (cb < ce) {
-      for (; s < k && dd.fid[s] < ce; ++s) {
-        const uint32_t lo = max(dd.fid[s], cb), hi = min(dd.fid[s + 1], ce);
-        const CT lws = (CT)lwsrc[s];
-        double acc[NV];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-        for (uint32_t base = lo; base < hi; base += kWave) {
-          const uint32_t lim = min(base + (uint32_t)kWave, hi);
-          // merge the row's positive counts through LDS
-          uint64_t e = ip + (uint64_t)lane;
-          uint32_t c = e < g1 ? dd.gcell[e] : 0xffffffffu;
-          bool inrow = e < g1 && c < lim;
-          int m = popc64(wave_ballot(inrow));
-          sh.ysc[wave][lane] = 0.0;
-          wave_lds_sync();
-          if (inrow) sh.ysc[wave][c - base] = sizeof(CT) == 4 ? (double)dd.gy32[e] : dd.gy64[e];
-          wave_lds_sync();
-          const CT y = (CT)sh.ysc[wave][lane];
-          ip += (uint64_t)m;
-          const uint32_t cell = base + (uint32_t)lane;
-          if (cell < lim) {
-            CT x[NB];
-            CT eta;
-            if constexpr (sizeof(CT) == 4) {
-              const float* row = dd.x32 + (size_t)cell * dd.p4;
-#pragma unroll
-              for (int j = 0; j < NB; ++j) x[j] = row[j];
-              eta = dd.off32[cell];
-            } else {
-              const double* row = dd.x64 + (size_t)cell * NB;
-#pragma unroll
-              for (int j = 0; j < NB; ++j) x[j] = row[j];
-              eta = dd.off64[cell];
-            }
-#pragma unroll
-            for (int j = 0; j < NB; ++j) eta += x[j] * b[j];
-            const CT ex = dexp<CT>(eta + lws);
-            const CT t = ex + g;
-            const CT inv = drcp<CT>(t);
-            const CT u = ex * inv;
-            const CT gs = g + y;
-            if constexpr (MODE == SW_LAPLACE) {
-              // (gamma + y) times the four derivative factors (pml.rs:1861-1874)
-              double e64 = (double)ex, t64 = e64 + gamma, gs64 = gamma + (double)y;
-              double p2 = e64 / (t64 * t64), p3 = p2 / t64, p4 = p3 / t64, p5 = p4 / t64;
-              double e2 = e64 * e64;
-              acc[0] += gs64 * p2;
-              acc[1] += gs64 * (p3 * (gamma - e64));
-              acc[2] += gs64 * (p4 * (gamma * gamma + e2 - (4.0 * gamma) * e64));
-              acc[3] += gs64 * (p5 * (gamma * gamma * gamma - (11.0 * gamma * gamma) * e64 +
-                                      (11.0 * gamma) * e2 - e2 * e64));
-            } else {
-              const CT lt = dlog<CT>(t);
-              const double pw = (double)(gs * u * inv);
-              const double term = (double)(y * eta) - (double)gs * (double)lt;
-              if constexpr (MODE == SW_VALUE) {
-                acc[0] += term;
-                acc[1] += pw;
-              } else {
-                const double dres = (double)y - (double)gs * (double)u;
-                acc[0] += dres;
-                acc[1] += pw;
-                acc[2] += term;
-                double xd[NB];
-#pragma unroll
-                for (int j = 0; j < NB; ++j) xd[j] = (double)x[j];
-#pragma unroll
-                for (int a = 0; a < NB; ++a) {
-                  acc[3 + a] += xd[a] * dres;
-                  const double pa = pw * xd[a];
-                  acc[3 + NB + a] += pa;
-#pragma unroll
-                  for (int bb = a; bb < NB; ++bb)
-                    acc[3 + 2 * NB + a * NB - a * (a - 1) / 2 + (bb - a)] += pa * xd[bb];
-                }
-              }
-            }
-          }
-        }
-        // flush subject s: wave sum, slot s + wave
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          double r = wave_sum(acc[v]);
-          if (lane == 0) sc.part[(size_t)(s + wave) * V + v] = r;
-        }
-      }
-    }
+  __device__ __noinline__ void sweep(const double* bsrc, const double* lwsrc) {
+    constexpr int NSV = MODE == SW_LAPLACE ? 4 : NS;
+    constexpr int NGV = MODE == SW_LAPLACE ? 0 : NG;
+    const CT* rows = sizeof(CT) == 4 ? (const CT*)dd.row32 : (const CT*)dd.row64;
+    const CT* gy = sizeof(CT) == 4 ? (const CT*)dd.gy32 : (const CT*)dd.gy64;
+    sweep_wave<CT, MODE, NB>(rows, dd.gcell, gy, dd.fid, bsrc, lwsrc,
+                             (typename Lds<CT>::ptr)(&sh.ysc[wave][0]), wk.psub, wk.pgene,
+                             sh.ip0[wave], g1, (int)sh.s0[wave], k, block_begin(wave),
+                             block_end(wave), icpt, gamma, lane, wave);
     __syncthreads();
-    // per-subject totals, waves added in order
-    for (int idx = tid; idx < k * NV; idx += NT) {
-      const int ss = idx / NV, v = idx % NV;
+    // per-subject totals, waves added in order; per-gene totals likewise
+    for (int idx = tid; idx < k * NSV; idx += NT) {
+      const int ss = idx / NSV, v = idx % NSV;
       const uint32_t f0 = dd.fid[ss], f1 = dd.fid[ss + 1];
       const int w0 = (int)(f0 / blk), w1 = (int)((f1 - 1) / blk);
       double t = 0.0;
-      for (int w = w0; w <= w1; ++w) t += sc.part[(size_t)(ss + w) * V + v];
-      sc.tot[(size_t)ss * V + v] = t;
+      for (int w = w0; w <= w1; ++w) t += wk.psub[(size_t)(ss + w) * NSL + v];
+      wk.tsub[(size_t)ss * NSL + v] = t;
+    }
+    if constexpr (NGV > 0) {
+      for (int v = tid; v < NGV; v += NT) {
+        double t = 0.0;
+        for (int w = 0; w < WAVES; ++w) t += wk.pgene[(size_t)w * NG + v];
+        wk.tgene[v] = t;
+      }
     }
     if (tid == 0) {
       if (sizeof(CT) == 4) {
@@ -432,87 +597,118 @@ struct Fitter {
     __syncthreads();
   }
 
+  // Sum over subjects of f(s), one wave per quantity: lane l takes subjects
+  // l, l + 64, ...; the wave's fixed butterfly adds the lanes.
+  template <class F>
+  __device__ double subject_sum(F f) const {
+    double acc = 0.0;
+    for (int s = lane; s < k; s += kWave) acc += f(s);
+    return wave_sum(acc);
+  }
+
   // Gradient, curvature blocks and penalised log-likelihood of side `sd` from
-  // sc.tot (pml.rs:1564-1589), log w in `lwsrc`.
-  __device__ void finalize(int sd, const double* lwsrc) {
+  // the sweep totals (pml.rs:1564-1589), log w in `lwsrc`.
+  __device__ __noinline__ void finalize(int sd, const double* lwsrc) {
     const double* tot_y = dd.gtot + (size_t)gene * k;
-    double* sub = sc.sub[sd];
+    double* sub = side(sd);
     for (int s = tid; s < k; s += NT) {
-      const double* t = sc.tot + (size_t)s * V;
+      const double* t = wk.tsub + (size_t)s * NSL;
       const double lws = lwsrc[s];
       const double w = exp(lws);
       sub[s * SUB + 0] = t[0] + (alpha - lambda * w);
       sub[s * SUB + 1] = gamma * t[1] + lambda * w;
       sub[s * SUB + 2] = w;
+      sub[s * SUB + 3 + icpt] = gamma * t[1];
 #pragma unroll
-      for (int a = 0; a < NB; ++a) sub[s * SUB + 3 + a] = gamma * t[3 + NB + a];
-      sc.llsub[s] = t[2] + lws * tot_y[s] + (alpha * lws - lambda * w);
+      for (int q = 0; q < Q; ++q) sub[s * SUB + 3 + qcol(q, icpt)] = gamma * t[2 + q];
+      wk.llsub[s] = lws * tot_y[s] + (alpha * lws - lambda * w);
     }
     __syncthreads();
-    // db (NB), vb (NTRI), ll: one thread each, subjects in order
-    for (int idx = tid; idx < NB + NTRI + 1; idx += NT) {
-      double acc = 0.0;
-      if (idx < NB) {
-        for (int s = 0; s < k; ++s) acc += sc.tot[(size_t)s * V + 3 + idx];
-        sh.db[sd][idx] = acc;
-      } else if (idx < NB + NTRI) {
-        const int q = idx - NB;
-        int a = 0, r = q;
+    // quantities summed over subjects: 0 resid, 1 weight, 2..1+Q first, 2+Q ll
+    for (int e = wave; e < Q + 3; e += WAVES) {
+      double v;
+      if (e < 2 + Q)
+        v = subject_sum([&](int s) { return wk.tsub[(size_t)s * NSL + e]; });
+      else
+        v = subject_sum([&](int s) { return wk.llsub[s]; });
+      if (lane == 0) {
+        if (e == 0) {
+          sh.db[sd][icpt] = v;
+        } else if (e == 1) {
+          sh.vb[sd][icpt * NB + icpt] = gamma * v;
+        } else if (e < 2 + Q) {
+          const int j = qcol(e - 2, icpt);
+          sh.vb[sd][icpt * NB + j] = gamma * v;
+          sh.vb[sd][j * NB + icpt] = gamma * v;
+        } else {
+          sh.ll[sd] = wk.tgene[0] + v;
+        }
+      }
+    }
+    for (int idx = tid; idx < Q + QT; idx += NT) {
+      if (idx < Q) {
+        sh.db[sd][qcol(idx, icpt)] = wk.tgene[1 + idx];
+      } else {
+        const int tq = idx - Q;
+        int a = 0, r = tq;
+        while (r >= Q - a) {
+          r -= Q - a;
+          ++a;
+        }
+        const int b = a + r;
+        const double v = gamma * wk.tgene[1 + Q + tq];
+        const int ja = qcol(a, icpt), jb = qcol(b, icpt);
+        sh.vb[sd][ja * NB + jb] = v;
+        sh.vb[sd][jb * NB + ja] = v;
+      }
+    }
+    __syncthreads();
+  }
+
+  // Schur complement of side sd into sh.vb2 (and the right-hand side into
+  // sh.rhs when `with_rhs`), one wave per entry.
+  __device__ __noinline__ void schur(int sd, bool with_rhs) {
+    const double* sub = side(sd);
+    const int n_e = NTRI + (with_rhs ? NB : 0);
+    for (int e = wave; e < n_e; e += WAVES) {
+      if (e < NTRI) {
+        int a = 0, r = e;
         while (r >= NB - a) {
           r -= NB - a;
           ++a;
         }
-        const int bb = a + r;
-        for (int s = 0; s < k; ++s) acc += gamma * sc.tot[(size_t)s * V + 3 + 2 * NB + q];
-        sh.vb[sd][a * NB + bb] = acc;
-        sh.vb[sd][bb * NB + a] = acc;
+        const int b = a + r;
+        const double acc = subject_sum([&](int s) {
+          return sub[s * SUB + 3 + a] * sub[s * SUB + 3 + b] / sub[s * SUB + 1];
+        });
+        if (lane == 0) {
+          const double v = sh.vb[sd][a * NB + b] - acc;
+          sh.vb2[a * NB + b] = v;
+          sh.vb2[b * NB + a] = v;
+        }
       } else {
-        for (int s = 0; s < k; ++s) acc += sc.llsub[s];
-        sh.ll[sd] = acc;
+        const int j = e - NTRI;
+        const double acc = subject_sum([&](int s) {
+          return sub[s * SUB + 3 + j] * (sub[s * SUB + 0] / sub[s * SUB + 1]);
+        });
+        if (lane == 0) sh.rhs[j] = sh.db[sd][j] - acc;
       }
     }
     __syncthreads();
   }
 
   // Newton step from side `sd` (pml.rs:1664-1710): Schur complement, LDL'
-  // solve, per-subject steps; writes sh.new_beta and sc.new_lw.
-  __device__ void newton_step(int sd) {
-    const double* sub = sc.sub[sd];
-    for (int idx = tid; idx < NTRI + NB; idx += NT) {
-      if (idx < NTRI) {
-        int a = 0, r = idx;
-        while (r >= NB - a) {
-          r -= NB - a;
-          ++a;
-        }
-        const int bb = a + r;
-        double acc = 0.0;
-        for (int s = 0; s < k; ++s)
-          acc += sub[s * SUB + 3 + a] * sub[s * SUB + 3 + bb] / sub[s * SUB + 1];
-        const double v = sh.vb[sd][a * NB + bb] - acc;
-        sh.vb2[a * NB + bb] = v;
-        sh.vb2[bb * NB + a] = v;
-      } else {
-        const int j = idx - NTRI;
-        double acc = 0.0;
-        for (int s = 0; s < k; ++s)
-          acc += sub[s * SUB + 3 + j] * (sub[s * SUB + 0] / sub[s * SUB + 1]);
-        sh.rhs[j] = sh.db[sd][j] - acc;
-      }
-    }
-    __syncthreads();
+  // solve, per-subject steps; writes sh.new_beta and wk.new_lw.
+  __device__ __noinline__ void newton_step(int sd) {
+    const double* sub = side(sd);
+    schur(sd, true);
     if (tid == 0) {
-      double f[NB * NB];
-#pragma unroll
-      for (int i = 0; i < NB * NB; ++i) f[i] = sh.vb2[i];
-      double st[NB];
-#pragma unroll
-      for (int i = 0; i < NB; ++i) st[i] = sh.rhs[i];
-      ldlt_solve_dev<NB>(f, st);
-#pragma unroll
+      for (int i = 0; i < NB * NB; ++i) sh.lm[i] = sh.vb2[i];
+      for (int i = 0; i < NB; ++i) sh.lb[i] = sh.rhs[i];
+      ldlt_solve_lds<NB>(sh.lm, sh.lb, sh.perm);
       for (int i = 0; i < NB; ++i) {
-        sh.step_beta[i] = st[i];
-        sh.new_beta[i] = sh.beta[i] + st[i];
+        sh.step_beta[i] = sh.lb[i];
+        sh.new_beta[i] = sh.beta[i] + sh.lb[i];
         sh.damp_beta[i] = 1.0;
       }
     }
@@ -523,16 +719,16 @@ struct Fitter {
       for (int j = 0; j < NB; ++j) acc += sub[s * SUB + 3 + j] * sh.step_beta[j];
       const double dwvw = sub[s * SUB + 0] / sub[s * SUB + 1];
       const double stp = dwvw - acc / sub[s * SUB + 1];
-      sc.step_lw[s] = stp;
-      sc.new_lw[s] = sc.lw[s] + stp;
-      sc.damp_lw[s] = 1.0;
-      sc.vw_step[s] = sub[s * SUB + 1];
+      wk.step_lw[s] = stp;
+      wk.new_lw[s] = wk.lw[s] + stp;
+      wk.damp_lw[s] = 1.0;
+      wk.vw_step[s] = sub[s * SUB + 1];
     }
     __syncthreads();
   }
 
   // Per-coordinate damping of the trial point (pml.rs:1729-1755).
-  __device__ void damp(double min_step) {
+  __device__ __noinline__ void damp(double min_step) {
     const double cut = 40.0;
     if (tid < NB) {
       const int j = tid;
@@ -547,32 +743,32 @@ struct Fitter {
       }
     }
     for (int s = tid; s < k; s += NT) {
-      const double d = sc.step_lw[s];
+      const double d = wk.step_lw[s];
       if (d < cut && d > -cut) {
-        sc.damp_lw[s] /= 2.0;
-        sc.new_lw[s] = sc.lw[s] + d * sc.damp_lw[s];
+        wk.damp_lw[s] /= 2.0;
+        wk.new_lw[s] = wk.lw[s] + d * wk.damp_lw[s];
       } else if (d > 0.0) {
-        sc.new_lw[s] = sc.lw[s] + min_step;
+        wk.new_lw[s] = wk.lw[s] + min_step;
       } else {
-        sc.new_lw[s] = sc.lw[s] - min_step;
+        wk.new_lw[s] = wk.lw[s] - min_step;
       }
     }
     __syncthreads();
   }
 
-  __device__ void accept() {
+  __device__ __noinline__ void accept() {
     if (tid < NB) sh.beta[tid] = sh.new_beta[tid];
-    for (int s = tid; s < k; s += NT) sc.lw[s] = sc.new_lw[s];
+    for (int s = tid; s < k; s += NT) wk.lw[s] = wk.new_lw[s];
     __syncthreads();
   }
 
   // max |gradient| over db and dw of side sd (pml.rs:1720-1724)
-  __device__ double max_grad(int sd) {
+  __device__ __noinline__ double max_grad(int sd) {
     if (tid == 0) {
       double m = 0.0;
       for (int j = 0; j < NB; ++j) m = fmax(m, fabs(sh.db[sd][j]));
       double mw = 0.0;
-      for (int s = 0; s < k; ++s) mw = fmax(mw, fabs(sc.sub[sd][s * SUB + 0]));
+      for (int s = 0; s < k; ++s) mw = fmax(mw, fabs(side(sd)[s * SUB + 0]));
       sh.scal[0] = fmax(m, mw) > 0.01 ? 1.0 : 0.0;
     }
     __syncthreads();
@@ -587,8 +783,8 @@ struct Fitter {
   template <class CT>
   __device__ int newton_loop(const FitKnobs& kn, double noise_rel, FitResult& out) {
     int cur = 0;
-    sweep<CT, SW_FULL>(sh.beta, sc.lw);
-    finalize(cur, sc.lw);
+    sweep<CT, SW_FULL>(sh.beta, wk.lw);
+    finalize(cur, wk.lw);
     double ll = sh.ll[cur], ll_prev = 0.0, likdif = 0.0;
     int step = 0, backtracks = 0;
     while (step == 0 ||
@@ -598,8 +794,8 @@ struct Fitter {
       newton_step(cur);
       const int tr = 1 - cur;
       ll_prev = ll;
-      sweep<CT, SW_FULL>(sh.new_beta, sc.new_lw);
-      finalize(tr, sc.new_lw);
+      sweep<CT, SW_FULL>(sh.new_beta, wk.new_lw);
+      finalize(tr, wk.new_lw);
       ll = sh.ll[tr];
       likdif = ll - ll_prev;
       const double noise = noise_rel * fabs(ll_prev);
@@ -615,8 +811,8 @@ struct Fitter {
           break;
         }
         damp(min_step);
-        sweep<CT, SW_FULL>(sh.new_beta, sc.new_lw);
-        finalize(tr, sc.new_lw);
+        sweep<CT, SW_FULL>(sh.new_beta, wk.new_lw);
+        finalize(tr, wk.new_lw);
         ll = sh.ll[tr];
         likdif = ll - ll_prev;
       }
@@ -636,14 +832,14 @@ struct Fitter {
   // rounding or leaves it not finite, or the budget runs out.
   __device__ bool finish(const FitKnobs& kn, FitResult& out) {
     int cur = 0;
-    sweep<double, SW_FULL>(sh.beta, sc.lw);
-    finalize(cur, sc.lw);
+    sweep<double, SW_FULL>(sh.beta, wk.lw);
+    finalize(cur, wk.lw);
     for (int it = 1; it <= kn.max_iter; ++it) {
       const double ll_prev = sh.ll[cur];
       newton_step(cur);
       const int tr = 1 - cur;
-      sweep<double, SW_FULL>(sh.new_beta, sc.new_lw);
-      finalize(tr, sc.new_lw);
+      sweep<double, SW_FULL>(sh.new_beta, wk.new_lw);
+      finalize(tr, wk.new_lw);
       const double ll = sh.ll[tr];
       const double likdif = ll - ll_prev;
       if (isnan(likdif) || likdif < -1e-12 * fabs(ll)) return false;
@@ -653,7 +849,7 @@ struct Fitter {
         // log-det at the stepped point
         if (tid == 0) {
           double ld = 0.0;
-          for (int s = 0; s < k; ++s) ld += log(fabs(sc.sub[cur][s * SUB + 1]));
+          for (int s = 0; s < k; ++s) ld += log(fabs(side(cur)[s * SUB + 1]));
           sh.scal[1] = ld;
         }
         __syncthreads();
@@ -663,38 +859,18 @@ struct Fitter {
         out.finish_steps = it;
         out.second = 0.0;
         // information at the final point for the reply
-        newton_info(cur);
+        schur(cur, false);
         return true;
       }
     }
     return false;
   }
 
-  // Schur complement of side sd into sh.vb2 (no step), for the reply.
-  __device__ void newton_info(int sd) {
-    const double* sub = sc.sub[sd];
-    for (int idx = tid; idx < NTRI; idx += NT) {
-      int a = 0, r = idx;
-      while (r >= NB - a) {
-        r -= NB - a;
-        ++a;
-      }
-      const int bb = a + r;
-      double acc = 0.0;
-      for (int s = 0; s < k; ++s)
-        acc += sub[s * SUB + 3 + a] * sub[s * SUB + 3 + bb] / sub[s * SUB + 1];
-      const double v = sh.vb[sd][a * NB + bb] - acc;
-      sh.vb2[a * NB + bb] = v;
-      sh.vb2[bb * NB + a] = v;
-    }
-    __syncthreads();
-  }
-
   // log-det from the last Newton step's vw (pml.rs:1767-1771)
-  __device__ double log_det_step() {
+  __device__ __noinline__ double log_det_step() {
     if (tid == 0) {
       double ld = 0.0;
-      for (int s = 0; s < k; ++s) ld += log(fabs(sc.vw_step[s]));
+      for (int s = 0; s < k; ++s) ld += log(fabs(wk.vw_step[s]));
       sh.scal[2] = ld;
     }
     __syncthreads();
@@ -702,14 +878,14 @@ struct Fitter {
   }
 
   // Laplace correction at the current point (pml.rs:1828-1919), f64.
-  __device__ double laplace(unsigned ord) {
-    sweep<double, SW_LAPLACE>(sh.beta, sc.lw);
+  __device__ __noinline__ double laplace(unsigned ord) {
+    sweep<double, SW_LAPLACE>(sh.beta, wk.lw);
     if (tid == 0) {
       const bool high = ord > 2;
       double third_acc = 0.0, fourth_acc = 0.0, fifth_acc = 0.0;
       for (int s = 0; s < k; ++s) {
-        const double* t = sc.tot + (size_t)s * V;
-        const double prior = lambda * exp(sc.lw[s]);
+        const double* t = wk.tsub + (size_t)s * NSL;
+        const double prior = lambda * exp(wk.lw[s]);
         const double vw = gamma * t[0] + prior;
         const double third = gamma * t[1] + prior;
         third_acc += third * third / (vw * vw * vw);
@@ -734,7 +910,7 @@ struct Fitter {
 
   // The whole fit.  mode 0: f32 Newton then f64 finish (stage two); mode 1:
   // opt_pml in f64 from the given point (stage three).  beta in sh.beta and
-  // log w in sc.lw on entry.
+  // log w in wk.lw on entry.
   __device__ FitResult run(int mode, unsigned ord, const FitKnobs& kn) {
     FitResult r{};
     r.finish_steps = 0;
@@ -750,13 +926,14 @@ struct Fitter {
     }
     if (!done) {
       FitResult e{};
-      newton_loop<double>(kn, 0.0, e);
+      const int cur = newton_loop<double>(kn, 0.0, e);
       r.ll = e.ll;
       r.ll_prev = e.ll_prev;
       r.iterations += e.iterations;
       r.backtracks = e.backtracks;
       r.log_det = log_det_step();
       r.exact = 1;
+      (void)cur;
     }
     r.second = ord > 1 ? laplace(ord) : 0.0;
     return r;

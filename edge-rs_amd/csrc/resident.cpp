@@ -26,16 +26,34 @@ void Resident::upload(size_t nc, size_t nb_, size_t k_, const double* design,
   if (!ev1) NH_HIP(hipEventCreate(&ev1));
   n_cells = nc;
   nb = nb_;
-  p4 = (nb + 3) / 4 * 4;
   k = k_;
   n_genes = ng;
-  std::vector<float> h32(n_cells * p4, 0.0f), ho32(n_cells);
-  for (size_t i = 0; i < n_cells; ++i) {
-    for (size_t j = 0; j < nb; ++j) h32[i * p4 + j] = (float)design[i * nb + j];
-    ho32[i] = (float)log_offset[i];
+  // the intercept: the first column that is 1 in every cell (nebula's centred
+  // design always has one); the fit keeps its sums implicitly
+  icpt = nb;
+  for (size_t j = 0; j < nb && icpt == nb; ++j) {
+    bool ones = true;
+    for (size_t i = 0; i < n_cells && ones; ++i) ones = design[i * nb + j] == 1.0;
+    if (ones) icpt = j;
   }
-  x32.upload(h32.data(), h32.size(), stream);
-  off32.upload(ho32.data(), ho32.size(), stream);
+  if (icpt == nb)
+    throw std::invalid_argument("the design needs an intercept column (1 in every cell)");
+  rw = packed_row_width((uint32_t)nb);
+  std::vector<float> h32(n_cells * rw, 0.0f);
+  std::vector<double> h64(n_cells * rw, 0.0);
+  for (size_t i = 0; i < n_cells; ++i) {
+    size_t q = 0;
+    for (size_t j = 0; j < nb; ++j) {
+      if (j == icpt) continue;
+      h32[i * rw + q] = (float)design[i * nb + j];
+      h64[i * rw + q] = design[i * nb + j];
+      ++q;
+    }
+    h32[i * rw + q] = (float)log_offset[i];
+    h64[i * rw + q] = log_offset[i];
+  }
+  row32.upload(h32.data(), h32.size(), stream);
+  row64.upload(h64.data(), h64.size(), stream);
   x64.upload(design, n_cells * nb, stream);
   off64.upload(log_offset, n_cells, stream);
   std::vector<uint32_t> hf(k + 1);
@@ -65,9 +83,9 @@ void Resident::upload(size_t nc, size_t nb_, size_t k_, const double* design,
 
 DeviceData Resident::view() const {
   DeviceData d;
-  d.x32 = x32.p;
+  d.row32 = row32.p;
+  d.row64 = row64.p;
   d.x64 = x64.p;
-  d.off32 = off32.p;
   d.off64 = off64.p;
   d.fid = fid.p;
   d.gptr = gptr.p;
@@ -77,8 +95,9 @@ DeviceData Resident::view() const {
   d.gtot = gtot.p;
   d.n_cells = (uint32_t)n_cells;
   d.nb = (uint32_t)nb;
-  d.p4 = (uint32_t)p4;
   d.k = (uint32_t)k;
+  d.icpt = (uint32_t)icpt;
+  d.rw = rw;
   return d;
 }
 
@@ -116,8 +135,9 @@ void Resident::solve(const std::vector<DevRequest>& r, const std::vector<double>
   out_beta.alloc(n * nb);
   out_lw.alloc(n * k);
   out_info.alloc(n * nb * nb);
-  const size_t stride = fit_scratch_doubles((int)nb, (int)k, waves);
-  scratch.alloc(stride * n);
+  const size_t stride = fit_work_doubles((int)nb, (int)k, waves);
+  const bool in_lds = stride * sizeof(double) <= kWorkLdsBytes;
+  if (!in_lds) scratch.alloc(stride * n);
   BatchLaunch L;
   L.dd = view();
   L.req = req.p;
@@ -127,8 +147,9 @@ void Resident::solve(const std::vector<DevRequest>& r, const std::vector<double>
   L.out_beta = out_beta.p;
   L.out_lw = out_lw.p;
   L.out_info = out_info.p;
-  L.scratch = scratch.p;
+  L.scratch = in_lds ? nullptr : scratch.p;
   L.stride = stride;
+  L.work_in_lds = in_lds ? 1 : 0;
   L.knobs.eps = eps;
   L.knobs.noise_rel = 1e-8;
   L.knobs.max_iter = 50;

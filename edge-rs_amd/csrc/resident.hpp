@@ -52,8 +52,10 @@ struct DevBuf {
 struct Resident {
   int device = 0;
   hipStream_t stream = nullptr;
-  size_t n_cells = 0, nb = 0, p4 = 0, k = 0, n_genes = 0;
-  DevBuf<float> x32, off32, gy32;
+  size_t n_cells = 0, nb = 0, k = 0, n_genes = 0, icpt = 0;
+  uint32_t rw = 1;  // values per packed cell row
+  DevBuf<float> row32, gy32;
+  DevBuf<double> row64;
   DevBuf<double> x64, off64, gy64, gtot;
   DevBuf<uint32_t> fid, gcell;
   DevBuf<uint64_t> gptr;

@@ -229,7 +229,8 @@ enum {
   NH_SOLVE_F64 = 1         /* opt_pml in FP64 on the device (stage three) */
 };
 
-/* Uploads the shared design (n_cells * n_coef, already centred, f64), log
+/* Uploads the shared design (n_cells * n_coef, already centred, f64, with an
+ * intercept column that is 1 in every cell, else NH_ERR_INVALID_ARGUMENT), log
  * offsets, subject boundaries (n_subjects + 1) and per-gene positive counts
  * (CSR over genes: gene_ptr n_genes+1, cells, counts) once. */
 int nh_resident_upload(size_t n_cells, size_t n_coef, size_t n_subjects,
