@@ -1,7 +1,8 @@
-"""NEBULA-HL stage two on one MI355X: genes/sec of the batched engine
-(persistent work-queue kernel, search on the device, hipGraph launch) against
-the plain engine (host BOBYQA, one fit launch per search round), at the
-scaled workload of 500 genes x 100,000 cells x 20 subjects, p = 4.
+"""NEBULA-HL stage two on one MI355X: genes/sec of the learned + batched engine
+(MLP warm start, persistent work-queue kernel, search on the device, hipGraph
+launch), with the plain engine (host BOBYQA, one fit launch per search round)
+and the cold batched engine measured beside it, on the workload
+500 genes x 100,000 cells x 20 subjects, p = 4.
 
 One step = stage two over every searched gene of the workload, inputs resident
 in HBM (set-up, upload and stage one happen before the timed region).
@@ -22,11 +23,23 @@ sys.path.insert(0, os.path.join(ROOT, "edge-rs_amd"))
 import numpy as np  # noqa: E402
 
 METRIC = "genes/sec NEBULA-HL stage 2, learned+batched vs plain HIP, |Δloglik|<1e-6 rel"
+WORKLOAD = "nebula_hl_stage_two_500g_100kc_20s_p4"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC = os.path.join(ROOT, "profiles", "stage_two_kernel_traffic.json")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def device_sync():
+    """Device-wide synchronisation on both sides of the timed region."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except ImportError:
+        pass
 
 
 def algorithmic_bytes(st, p):
@@ -38,12 +51,15 @@ def algorithmic_bytes(st, p):
     return b32 + b64
 
 
-def read_traffic(kernel, workload):
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
-    if os.path.exists(path):
-        with open(path) as fh:
-            return json.load(fh).get("hbm_bytes_per_launch")
-    return None
+def measured_traffic(workload, engine):
+    """HBM bytes per launch of the persistent kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary, if it covers this workload."""
+    if not os.path.exists(TRAFFIC):
+        return None
+    with open(TRAFFIC) as fh:
+        t = json.load(fh)
+    rec = t.get(workload, {}).get(engine)
+    return rec.get("hbm_bytes_per_launch") if rec else None
 
 
 def copy_calibrate(mib=256, reps=4):
@@ -61,6 +77,14 @@ def copy_calibrate(mib=256, reps=4):
     log(f"copy: {reps} x {mib} MiB device-to-device")
 
 
+def timed(sess, engine, steps):
+    device_sync()
+    t0 = time.perf_counter()
+    runs = [sess.stage_two(engine) for _ in range(steps)]
+    device_sync()
+    return time.perf_counter() - t0, runs
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,13 +92,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--genes", type=int, default=500)
     ap.add_argument("--cells", type=int, default=100_000)
-    ap.add_argument("--cpu-sample", type=int, default=16,
+    ap.add_argument("--engine", choices=["learned", "batched", "plain"], default="learned",
+                    help="engine timed for `value` (only learned runs under rocprofv3 "
+                         "when profiling one kernel)")
+    ap.add_argument("--cpu-sample", type=int, default=48,
                     help="genes of the same workload timed on the CPU oracle")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--check", type=int, default=16,
+    ap.add_argument("--check", type=int, default=24,
                     help="genes whose minimisers are re-evaluated by the f64 oracle")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--no-compare", action="store_true",
+                    help="time only --engine (no plain / batched comparison runs)")
+    ap.add_argument("--compare-steps", type=int, default=2)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the workgroup-size autotuner (default heuristic)")
+    ap.add_argument("--weights", default=None, help="MLP parameter blob (.npy)")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="run only a 256 MiB device-to-device copy (the scale for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -91,6 +123,8 @@ def main():
     import nebula_hip as nh
     from nebula_hip import synth
 
+    engines = {"learned": nh.ENGINE_LEARNED, "batched": nh.ENGINE_BATCHED,
+               "plain": nh.ENGINE_PLAIN}
     t0 = time.time()
     w = synth.config(2, n_genes=a.genes, n_cells=a.cells)
     nnz_frac = np.diff(w.indptr.astype(np.int64)) / w.n_cells
@@ -99,53 +133,67 @@ def main():
         f"generated in {time.time() - t0:.1f}s")
     counts = nh.CompressedSparse(w.data, w.indices, w.indptr, (w.n_genes, w.n_cells))
     params = nh.NebulaParams(method="HL")
+    weights = nh.load_weights(a.weights) if a.weights else nh.load_weights()
     t0 = time.time()
-    sess = nh.Session(counts, w.subject_id, w.design, w.n_coef, w.offset, params)
+    sess = nh.Session(counts, w.subject_id, w.design, w.n_coef, w.offset, params,
+                      mlp_weights=weights)
     log(f"session: setup+upload+stage one {time.time() - t0:.2f}s, kept {sess.n_kept}")
 
-    # Each stage_two call returns after its stream has synchronised, so the
-    # timed region is bracketed by device synchronisation on both sides.
+    eng = engines[a.engine]
+    tune = None
+    if not a.no_tune and a.engine != "plain":
+        picked, ms = sess.autotune(eng)
+        tune = {"threads_per_gene": picked, "ms_by_threads": {str(k): round(v, 2)
+                                                              for k, v in ms.items()}}
+        log(f"autotune: {tune}")
     for i in range(a.warmup):
-        r = sess.stage_two(nh.ENGINE_BATCHED)
+        r = sess.stage_two(eng)
         log(f"warmup {i}: {r['stats']['t_stage_two']:.4f}s")
-    t0 = time.perf_counter()
-    runs = []
-    for i in range(a.steps):
-        runs.append(sess.stage_two(nh.ENGINE_BATCHED))
-    t_batched = time.perf_counter() - t0
+    t_main, runs = timed(sess, eng, a.steps)
     st = runs[-1]["stats"]
     searched = int(st["genes_searched"])
-    ms_per_step = 1e3 * t_batched / a.steps
-    value = searched * a.steps / t_batched
-    log(f"batched: {ms_per_step:.2f} ms/step, {value:.1f} genes/s, evals {st['evaluations']}, "
-        f"kernel {st['fit_kernel_ms']:.2f} ms")
+    ms_per_step = 1e3 * t_main / a.steps
+    value = searched * a.steps / t_main
+    log(f"{a.engine}: {ms_per_step:.2f} ms/step, {value:.1f} genes/s, "
+        f"evals/gene {st['evaluations'] / max(1, searched):.2f}, "
+        f"fallbacks {st['warm_fallbacks']}, kernel {st['fit_kernel_ms']:.2f} ms")
 
-    # plain engine at the same inputs
-    t0 = time.perf_counter()
-    plain_runs = [sess.stage_two(nh.ENGINE_PLAIN) for _ in range(a.plain_steps)]
-    t_plain = time.perf_counter() - t0
-    plain_value = searched * a.plain_steps / t_plain
-    pst = plain_runs[-1]["stats"]
-    log(f"plain: {1e3 * t_plain / a.plain_steps:.2f} ms/step, {plain_value:.1f} genes/s, "
-        f"rounds {pst['rounds']}, kernel {pst['fit_kernel_ms']:.2f} ms")
-
-    # roofline of the dominant kernel (the persistent stage-two kernel)
-    bytes_alg = algorithmic_bytes(st, w.n_coef)
-    kern_s = st["fit_kernel_ms"] * 1e-3 / max(1, st["fit_kernel_launches"])
-    achieved = bytes_alg / max(1, st["fit_kernel_launches"]) / kern_s / 1e9
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": read_traffic("stage_two_kernel", "config2"),
-            "kernel": "stage_two_kernel", "kernel_ms": round(st["fit_kernel_ms"], 3),
-            "algorithmic_bytes_per_launch": int(bytes_alg / max(1, st["fit_kernel_launches"]))}
-
-    extra = {"plain_genes_per_s": round(plain_value, 2),
-             "batched_over_plain": round(value / plain_value, 3),
+    extra = {"engine_timed": a.engine, "autotune": tune,
              "evaluations_per_gene": round(st["evaluations"] / max(1, searched), 2),
+             "warm_fallbacks": int(st["warm_fallbacks"]),
              "sweeps_f32": st["sweeps_f32"], "sweeps_f64": st["sweeps_f64"],
              "t_stage_one_s": round(st["t_stage_one"], 3)}
+    results = {a.engine: runs[-1]}
+    if not a.no_compare:
+        for name in ("batched", "plain"):
+            if name == a.engine:
+                continue
+            t_c, cr = timed(sess, engines[name], a.compare_steps)
+            results[name] = cr[-1]
+            rate = searched * a.compare_steps / t_c
+            cst = cr[-1]["stats"]
+            extra[f"{name}_genes_per_s"] = round(rate, 2)
+            extra[f"{name}_evaluations_per_gene"] = round(
+                cst["evaluations"] / max(1, searched), 2)
+            extra[f"{a.engine}_over_{name}"] = round(value / rate, 3)
+            log(f"{name}: {1e3 * t_c / a.compare_steps:.2f} ms/step, {rate:.1f} genes/s, "
+                f"evals/gene {cst['evaluations'] / max(1, searched):.2f}, "
+                f"kernel {cst['fit_kernel_ms']:.2f} ms")
 
-    # accuracy: oracle f64 objective at batched vs plain minimisers
+    # roofline of the dominant kernel (the persistent stage-two kernel, or the
+    # per-round fit kernel for the plain engine)
+    bytes_alg = algorithmic_bytes(st, w.n_coef)
+    launches = max(1, st["fit_kernel_launches"])
+    kern_s = st["fit_kernel_ms"] * 1e-3 / launches
+    achieved = bytes_alg / launches / kern_s / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(WORKLOAD, a.engine),
+            "kernel": "pml_batch_kernel" if a.engine == "plain" else "stage_two_kernel",
+            "kernel_ms_per_launch": round(1e3 * kern_s, 3),
+            "algorithmic_bytes_per_launch": int(bytes_alg / launches)}
+
+    # accuracy: the oracle's f64 profile objective at each engine's minimisers
     cpu = None
     try:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -153,13 +201,17 @@ def main():
         prm = oracle.params(method="HL")
         n = min(a.check, sess.n_kept)
         gi = np.arange(n)
-        mb = runs[-1]["minimisers"][:n]
-        mp = plain_runs[-1]["minimisers"][:n]
-        vb = oracle.profile_objective(w.indptr, w.indices, w.data, w.n_cells, w.subject_id,
-                                      w.design, w.n_coef, w.offset, prm, gi, mb)
-        vp = oracle.profile_objective(w.indptr, w.indices, w.data, w.n_cells, w.subject_id,
-                                      w.design, w.n_coef, w.offset, prm, gi, mp)
-        extra["dloglik_batched_vs_plain_max"] = float(np.max(np.abs(vb - vp) / np.abs(vp)))
+
+        def ll(mins, ip=w.indptr, ix=w.indices, dv=w.data, genes=gi):
+            return oracle.profile_objective(ip, ix, dv, w.n_cells, w.subject_id, w.design,
+                                            w.n_coef, w.offset, prm, genes, mins[: len(genes)])
+
+        v = {name: ll(r["minimisers"]) for name, r in results.items()}
+        if "plain" in v:
+            for name in v:
+                if name != "plain":
+                    extra[f"dloglik_{name}_vs_plain_max"] = float(
+                        np.max(np.abs(v[name] - v["plain"]) / np.abs(v["plain"])))
         if not a.no_cpu and a.cpu_sample > 0:
             threads = a.cpu_threads or min(16, os.cpu_count() or 1)
             m = min(a.cpu_sample, w.n_genes)
@@ -172,33 +224,31 @@ def main():
             wall = time.perf_counter() - t0
             t2 = ref["t_stage_two"]  # summed over threads
             n_ref = len(ref["gene_index"])
-            cpu_rate = n_ref / (t2 / min(threads, n_ref))
-            cpu = {"value": round(cpu_rate, 3), "unit": "genes/sec", "cores": min(threads, n_ref),
+            used = min(threads, n_ref)
+            cpu_rate = n_ref / (t2 / used)
+            cpu = {"value": round(cpu_rate, 3), "unit": "genes/sec", "cores": used,
                    "kind": "port",
-                   "sample": f"oracle nebula_sparse (f64 C++ restatement) on the first {m} genes "
-                             f"of the same workload, {min(threads, n_ref)} threads; stage-two "
-                             f"CPU time {t2:.2f}s summed over threads, wall {wall:.2f}s for all "
-                             f"three stages"}
-            # parity of the HIP minimisers against the CPU search on those genes:
+                   "sample": f"oracle nebula_sparse (f64 C++ restatement of the reference's "
+                             f"CPU path) on the first {m} genes of the same workload, {used} "
+                             f"threads; stage-two CPU time {t2:.2f}s summed over threads, "
+                             f"wall {wall:.2f}s for all three stages"}
             # the filter keeps genes in input order, so the first n_ref kept
             # genes of the full run are the sample's kept genes
-            vh = oracle.profile_objective(ip, ix, dv, w.n_cells, w.subject_id, w.design,
-                                          w.n_coef, w.offset, prm, np.arange(n_ref),
-                                          runs[-1]["minimisers"][:n_ref])
-            vc = oracle.profile_objective(ip, ix, dv, w.n_cells, w.subject_id, w.design,
-                                          w.n_coef, w.offset, prm, np.arange(n_ref),
-                                          ref["stage_two"][:, 1:3])
-            extra["dloglik_vs_cpu_max"] = float(np.max(np.abs(vh - vc) / np.abs(vc)))
+            g2 = np.arange(n_ref)
+            vh = ll(runs[-1]["minimisers"], ip, ix, dv, g2)
+            vc = ll(ref["stage_two"][:, 1:3], ip, ix, dv, g2)
+            extra[f"dloglik_{a.engine}_vs_cpu_max"] = float(np.max(np.abs(vh - vc) / np.abs(vc)))
     except Exception as e:  # the oracle is optional on a box without its build
         log(f"oracle checks skipped: {e}")
 
     out = {"metric": METRIC, "value": round(value, 2), "unit": "genes/sec", "n_gpus": 1,
            "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
-           "data": "synthetic (seeded NB-gamma mixed model, config-2 generator)",
-           "config": {"workload": "nebula_hl_stage_two_500g_100kc_20s_p4",
-                      "genes": w.n_genes, "cells": w.n_cells, "subjects": int(len(w.truth['sizes'])),
-                      "n_coef": w.n_coef, "genes_searched": searched, "engine": "batched"},
+           "data": "synthetic (seeded NB-gamma mixed model: Dirichlet(2) subject sizes, "
+                   "beta0 ~ N(-3, 1.5), offset = lib / mean(lib))",
+           "config": {"workload": WORKLOAD, "genes": w.n_genes, "cells": w.n_cells,
+                      "subjects": int(len(w.truth['sizes'])), "n_coef": w.n_coef,
+                      "genes_searched": searched, "engine": a.engine},
            "roofline": roof, "cpu_baseline": cpu}
     out.update(extra)
     print(json.dumps(out), flush=True)

@@ -115,16 +115,8 @@ struct Bobyqa {
                   const double* upper, double xtol_rel = 1e-6,
                   int maxeval_ = 1000) {
     NH_STRICT_FP
-    n = n_;
-    npt = 2 * n + 1;
-    ndim = npt + n;
-    maxeval = maxeval_;
-    nevals = 0;
-    want = false;
-    status = BOBYQA_RUNNING;
-    fbest_out = INFINITY;
     double dx[NMAX];
-    for (int i = 0; i < n; ++i) {
+    for (int i = 0; i < n_; ++i) {
       double lb = lower[i], ub = upper[i], xi = x0[i];
       double step = INFINITY;
       if (nh_finite(ub) && nh_finite(lb) && (ub - lb) * 0.25 < step && ub > lb)
@@ -140,6 +132,23 @@ struct Bobyqa {
       if (nh_inf(step) || step == 0.0) step = 1.0;
       dx[i] = step;
     }
+    init_with_step(n_, x0, lower, upper, dx, xtol_rel, maxeval_);
+  }
+
+  // The same set-up with the initial step given (NLopt's
+  // nlopt_set_initial_step instead of its default); rhoend = xtol_rel * rhobeg.
+  NH_HD void init_with_step(int n_, const double* x0, const double* lower,
+                            const double* upper, const double* dx, double xtol_rel,
+                            int maxeval_) {
+    NH_STRICT_FP
+    n = n_;
+    npt = 2 * n + 1;
+    ndim = npt + n;
+    maxeval = maxeval_;
+    nevals = 0;
+    want = false;
+    status = BOBYQA_RUNNING;
+    fbest_out = INFINITY;
     for (int i = 0; i <= n; ++i) s[i] = 1.0;
     bool differ = false;
     for (int i = 1; i < n; ++i) differ = differ || (dx[i] != dx[i - 1]);

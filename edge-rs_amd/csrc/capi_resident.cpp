@@ -46,6 +46,8 @@ void nebula_hip_options_default(nebula_hip_options_t* o) {
   o->wg_size = 0;
   o->device = 0;
   o->host_threads = 0;
+  o->learned_step = 0.0;
+  o->learned_xtol = 0.0;
 }
 
 int nh_resident_upload(size_t n_cells, size_t n_coef, size_t n_subjects, const double* design,

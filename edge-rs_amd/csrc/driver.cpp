@@ -625,6 +625,68 @@ void run_stage_one(Prepared& P, Resident& R, int host_threads, uint64_t& rounds)
 }
 
 // ---------------------------------------------------------------------------
+// Warm-start features of every kept gene, from its counts and stage one's
+// estimate (no counterpart in edge-rs):
+//   0 log(sum y / sum offset)           1 positive fraction nnz / N
+//   2 log(variance of y over cells)     3 log method-of-moments NB dispersion
+//   4-6 10/50/90 % quantiles of the per-subject log rate log((Y_s + 1/2) / O_s)
+//       minus feature 0                 7 their standard deviation
+//   8 log nnz                           9 log(N / K)
+//   10 log sigma^2 of stage one         11 log phi = -log gamma of stage one
+void compute_features(Prepared& P) {
+  const size_t G = P.kept.size(), k = P.k;
+  const nebula_params_t& p = P.params;
+  std::vector<double> osub(k, 0.0);
+  double otot = 0.0;
+  for (size_t s = 0; s < k; ++s) {
+    for (size_t c = P.fid[s]; c < P.fid[s + 1]; ++c) osub[s] += std::exp(P.log_offset[c]);
+    otot += osub[s];
+  }
+  const double n = (double)P.n_cells;
+  P.features.assign(G * 12, 0.0f);
+  std::vector<double> lr(k);
+  for (size_t i = 0; i < G; ++i) {
+    double sum = 0.0, sumsq = 0.0;
+    for (uint64_t e = P.gene_ptr[i]; e < P.gene_ptr[i + 1]; ++e) {
+      sum += P.counts[e];
+      sumsq += P.counts[e] * P.counts[e];
+    }
+    const double nnz = (double)(P.gene_ptr[i + 1] - P.gene_ptr[i]);
+    const double mean = sum / n, var = std::max(sumsq / n - mean * mean, 0.0);
+    const double rate = std::log(std::max(sum, 0.5) / otot);
+    const double mom = std::min(std::max((var - mean) / (mean * mean), 1e-4), 1e3);
+    for (size_t s = 0; s < k; ++s) lr[s] = std::log((P.genes[i].totals[s] + 0.5) / osub[s]) - rate;
+    std::vector<double> srt(lr);
+    std::sort(srt.begin(), srt.end());
+    auto quant = [&](double q) {
+      const double at = q * (double)(k - 1);
+      const size_t lo = (size_t)std::floor(at), hi = std::min(lo + 1, k - 1);
+      return srt[lo] + (at - (double)lo) * (srt[hi] - srt[lo]);
+    };
+    double m = 0.0, ss = 0.0;
+    for (double v : lr) m += v;
+    m /= (double)k;
+    for (double v : lr) ss += (v - m) * (v - m);
+    const GenePlan& pl = P.plans[i];
+    const double sig = std::min(std::max(pl.sigma, p.min_subject), p.max_subject);
+    const double gam = std::min(std::max(pl.gamma, p.min_cell), p.max_cell);
+    const double f[12] = {rate,
+                          nnz / n,
+                          std::log(var + 1e-8),
+                          std::log(mom),
+                          quant(0.1),
+                          quant(0.5),
+                          quant(0.9),
+                          std::sqrt(ss / (double)k),
+                          std::log(std::max(nnz, 1.0)),
+                          std::log(n / (double)k),
+                          std::log(sig),
+                          -std::log(gam)};
+    for (int j = 0; j < 12; ++j) P.features[i * 12 + j] = (float)f[j];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Stage two, plain engine: one BOBYQA per gene on the host, all live genes'
 // points in one fit launch per round (edge-rs search_all, stage_two.rs:362-544,
 // without the cohorts).  Inner fits start from the gene's first finite fit
@@ -843,6 +905,9 @@ struct nh_session {
   Prepared P;
   Resident R;
   nebula_hip_options_t o;
+  DevBuf<float> mlp;  // warm-start MLP parameters on the device
+  bool has_mlp = false;
+  int32_t tuned_wg = 0;  // autotuned threads per gene of the batched engines (0: none)
   double t_setup = 0, t_upload = 0, t_stage_one = 0;
   uint64_t s1_rounds = 0;
 };
@@ -876,6 +941,12 @@ int nh_session_create(size_t n_genes, size_t n_cells, const uint32_t* indptr,
                 S->P.gene_ptr.data(), S->P.cells.data(), S->P.counts.data(), S->o.device);
     const auto t2 = Clock::now();
     run_stage_one(S->P, S->R, S->o.host_threads, S->s1_rounds);
+    compute_features(S->P);
+    if (S->o.mlp_weights) {
+      S->mlp.upload(S->o.mlp_weights, nh_mlp_param_count(), S->R.stream);
+      NH_HIP(hipStreamSynchronize(S->R.stream));
+      S->has_mlp = true;
+    }
     const auto t3 = Clock::now();
     S->t_setup = secs(t0, t1);
     S->t_upload = secs(t1, t2);
@@ -892,6 +963,28 @@ int nh_session_create(size_t n_genes, size_t n_cells, const uint32_t* indptr,
 }
 
 size_t nh_session_genes(const nh_session_t* s) { return s ? s->P.kept.size() : 0; }
+
+int nh_session_features(const nh_session_t* s, float* out) {
+  if (!s || !out) return NH_ERR_INVALID_ARGUMENT;
+  std::memcpy(out, s->P.features.data(), s->P.features.size() * sizeof(float));
+  return NH_OK;
+}
+
+int nh_session_set_mlp(nh_session_t* s, const float* params, char* err, size_t err_len) {
+  if (!s || !params) {
+    put_err(err, err_len, "null argument");
+    return NH_ERR_INVALID_ARGUMENT;
+  }
+  try {
+    s->mlp.upload(params, nh_mlp_param_count(), s->R.stream);
+    NH_HIP(hipStreamSynchronize(s->R.stream));
+    s->has_mlp = true;
+  } catch (const std::exception& e) {
+    put_err(err, err_len, e.what());
+    return NH_ERR_GPU;
+  }
+  return NH_OK;
+}
 
 int nh_session_stage_one(const nh_session_t* s, double* sigma_gamma, int32_t* ord) {
   if (!s) return NH_ERR_INVALID_ARGUMENT;
@@ -920,13 +1013,22 @@ int nh_session_stage_two(nh_session_t* s, int32_t engine, double* minimisers,
       pl.v[0] = pl.v[1] = 0.0;
       pl.evaluations = 0;
       pl.objective = 0.0;
+      pl.warm_rejected = false;
     }
     s->R.reset_counters();
     s->R.kernel_ms = 0.0;
     s->R.launches = 0;
+    if (engine == NH_ENGINE_LEARNED && !s->has_mlp)
+      api_fail(NH_ERR_INVALID_ARGUMENT, "the learned engine needs MLP weights");
+    LearnedStart ls;
+    ls.params = s->mlp.p;
+    if (s->o.learned_step > 0.0) ls.step = s->o.learned_step;
+    if (s->o.learned_xtol > 0.0) ls.xtol_rel = s->o.learned_xtol;
     const auto t3 = Clock::now();
-    if (engine == NH_ENGINE_BATCHED)
-      run_stage_two_batched(s->P, s->R, s->o, st);
+    nebula_hip_options_t ob = s->o;
+    if (ob.wg_size == 0 && s->tuned_wg > 0) ob.wg_size = s->tuned_wg;
+    if (engine == NH_ENGINE_BATCHED || engine == NH_ENGINE_LEARNED)
+      run_stage_two_batched(s->P, s->R, ob, st, engine == NH_ENGINE_LEARNED ? &ls : nullptr);
     else
       run_stage_two_plain(s->P, s->R, waves_of(s->o), st);
     const auto t4 = Clock::now();
@@ -952,6 +1054,7 @@ int nh_session_stage_two(nh_session_t* s, int32_t engine, double* minimisers,
       }
       if (objectives) objectives[i] = ok ? pl.objective : NAN;
       if (evaluations) evaluations[i] = pl.evaluations;
+      if (pl.warm_rejected) ++st.warm_fallbacks;
     }
   } catch (const ApiError& e) {
     put_err(err, err_len, e.message);
@@ -961,6 +1064,31 @@ int nh_session_stage_two(nh_session_t* s, int32_t engine, double* minimisers,
     return NH_ERR_GPU;
   }
   if (stats) *stats = st;
+  return NH_OK;
+}
+
+int nh_session_autotune(nh_session_t* s, int32_t engine, int32_t* wg_size, double* ms, char* err,
+                        size_t err_len) {
+  if (!s) {
+    put_err(err, err_len, "null session");
+    return NH_ERR_INVALID_ARGUMENT;
+  }
+  const int32_t sizes[3] = {256, 512, 1024};
+  double best = INFINITY;
+  int32_t pick = 0;
+  for (int c = 0; c < 3; ++c) {
+    s->tuned_wg = sizes[c];
+    nebula_run_stats_t st;
+    const int rc = nh_session_stage_two(s, engine, nullptr, nullptr, nullptr, &st, err, err_len);
+    if (rc != NH_OK) return rc;
+    if (ms) ms[c] = 1e3 * st.t_stage_two;
+    if (st.t_stage_two < best) {
+      best = st.t_stage_two;
+      pick = sizes[c];
+    }
+  }
+  s->tuned_wg = pick;
+  if (wg_size) *wg_size = pick;
   return NH_OK;
 }
 
@@ -1007,7 +1135,8 @@ int nebula_sparse_hip_ex(size_t n_genes, size_t n_cells, const uint32_t* indptr,
   if (rc != NH_OK) return rc;
   nebula_run_stats_t st;
   std::memset(&st, 0, sizeof(st));
-  rc = nh_session_stage_two(s, s->o.engine, nullptr, nullptr, nullptr, &st, err, err_len);
+  const int32_t engine = s->o.learned ? NH_ENGINE_LEARNED : s->o.engine;
+  rc = nh_session_stage_two(s, engine, nullptr, nullptr, nullptr, &st, err, err_len);
   if (rc == NH_OK) rc = nh_session_finish(s, out, &st.t_stage_three, err, err_len);
   nh_session_free(s);
   st.t_total = secs(t0, Clock::now());

@@ -28,6 +28,7 @@ struct GenePlan {
   double v[2] = {0, 0};  // stage-two minimiser
   int evaluations = 0;
   double objective = 0;
+  bool warm_rejected = false;  // learned start fell back to the cold search
 };
 
 struct Prepared {
@@ -45,11 +46,20 @@ struct Prepared {
   std::vector<double> counts;
   std::vector<GeneInfo> genes;
   std::vector<GenePlan> plans;
+  std::vector<float> features;  // kept x 12 warm-start features (after stage one)
+};
+
+// Learned-start settings of one stage-two run.
+struct LearnedStart {
+  const float* params = nullptr;  // device copy of the MLP parameter blob
+  double step = 0.2;              // BOBYQA initial step, relative to the predicted point
+  double xtol_rel = 1e-3;         // rhoend / rhobeg of the warm search
 };
 
 double digamma(double x);
 double trigamma(double x);
+void compute_features(Prepared& P);
 void run_stage_two_batched(Prepared& P, Resident& R, const nebula_hip_options_t& o,
-                           nebula_run_stats_t& st);
+                           nebula_run_stats_t& st, const LearnedStart* learned);
 
 }  // namespace nh

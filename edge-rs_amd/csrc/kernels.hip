@@ -15,7 +15,8 @@ namespace nh {
 // (edge-rs ResidentBatch::solve, src/gpu/nebula_gpu.rs:441, one fit per plane
 // there; one per workgroup here).
 template <int NB, int WAVES>
-__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4))) void pml_batch_kernel(
+__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<NB>)))
+void pml_batch_kernel(
     DeviceData dd, const DevRequest* req, const double* beta_init, const double* lw_init,
     DevReply* out, double* out_beta, double* out_lw, double* out_info, double* scratch,
     size_t stride, FitKnobs kn, int mode, FitCounters* ctr, int work_in_lds) {
@@ -75,11 +76,14 @@ static hipError_t launch_batch_t(const BatchLaunch& L, hipStream_t st) {
   return hipGetLastError();
 }
 
+// 16 waves per gene only where a sweep fits 128 VGPRs (p <= 4).
 template <int NB>
 static hipError_t launch_batch_nb(const BatchLaunch& L, hipStream_t st) {
-  switch (L.waves) {
+  switch (effective_waves(NB, L.waves)) {
     case 4: return launch_batch_t<NB, 4>(L, st);
-    case 16: return launch_batch_t<NB, 16>(L, st);
+    case 16:
+      if constexpr (NB <= 4) return launch_batch_t<NB, 16>(L, st);
+      [[fallthrough]];
     default: return launch_batch_t<NB, 8>(L, st);
   }
 }

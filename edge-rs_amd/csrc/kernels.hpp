@@ -41,6 +41,12 @@ struct BatchLaunch {
 };
 
 size_t fit_work_doubles(int nb, int k, int waves);
+// Waves per gene a launch really uses: 16-wave workgroups need a sweep that
+// fits 128 VGPRs, which holds up to 4 design columns.
+inline int effective_waves(int nb, int waves) {
+  if (waves >= 16) return nb <= 4 ? 16 : 8;
+  return waves <= 4 ? 4 : 8;
+}
 // Largest per-workgroup work area placed in LDS (bytes); larger ones live in
 // global memory.
 constexpr size_t kWorkLdsBytes = 48 * 1024;

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <numeric>
 #include <vector>
@@ -28,6 +29,7 @@
 #include "driver.hpp"
 #include "pml_fit.hpp"
 #include "persistent.hpp"
+#include "mlp.hpp"
 #include "resident.hpp"
 
 namespace nh {
@@ -54,17 +56,26 @@ __device__ __noinline__ static void bq_init(Bobyqa<2>* b, const double* x0, cons
                                             const double* hi) {
   b->init(2, x0, lo, hi);
 }
+__device__ __noinline__ static void bq_init_step(Bobyqa<2>* b, const double* x0, const double* lo,
+                                                 const double* hi, const double* step,
+                                                 double xtol) {
+  b->init_with_step(2, x0, lo, hi, step, xtol, 1000);
+}
 __device__ __noinline__ static void bq_tell(Bobyqa<2>* b, double v) { b->tell(v); }
 
+enum SearchPhase { PH_COLD = 0, PH_WARM = 1, PH_CHECK = 2 };
+
 template <int NB, int WAVES>
-__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4))) void stage_two_kernel(PersistentArgs A) {
+__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<NB>)))
+void stage_two_kernel(PersistentArgs A) {
   constexpr int NT = WAVES * 64;
   __shared__ FitLds<NB, WAVES> sh;
   __shared__ Bobyqa<2> bq;
   __shared__ double warm_beta[NB];
   __shared__ double red[NT / 64];
   __shared__ int s_slot, s_active, s_found, s_failed, s_have_warm, s_evals, s_order, s_ok;
-  __shared__ double s_subject, s_cell, s_value;
+  __shared__ int s_phase, s_wfailed, s_rejected, s_from_warm;
+  __shared__ double s_subject, s_cell, s_value, s_wx[2], s_wf, s_cold[2], s_warm[2];
   extern __shared__ double nh_s2_dyn[];
   const int tid = threadIdx.x;
   const DeviceData& dd = A.dd;
@@ -73,6 +84,20 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4)))
   double* base = A.work_in_lds ? nh_s2_dyn : A.scratch + (size_t)blockIdx.x * A.stride;
   wk.carve(base, k, WAVES);
   double* warm_lw = base + FitWork<NB>::doubles(k, WAVES);
+  const double lo[2] = {A.lo[0], A.lo[1]}, hi[2] = {A.hi[0], A.hi[1]};
+  // (thread 0) starts a BOBYQA pass of the given phase at the current order
+  auto begin_search = [&](int phase) {
+    s_phase = phase;
+    if (phase == PH_WARM) {
+      double step[2];
+      for (int i = 0; i < 2; ++i)
+        step[i] = fmin(fmax(A.warm_step * s_warm[i], 1e-3 * lo[i]), 0.45 * (hi[i] - lo[i]));
+      bq_init_step(&bq, s_warm, lo, hi, step, A.warm_xtol);
+    } else {
+      bq_init(&bq, s_cold, lo, hi);
+    }
+    s_active = bq.status != BOBYQA_BAD_BOX && bq.want;
+  };
   for (;;) {
     if (tid == 0) s_slot = (int)atomicAdd(A.queue, 1u);
     __syncthreads();
@@ -86,17 +111,25 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4)))
       s_evals = 0;
       s_found = 0;
       s_failed = 0;
-      const double lo[2] = {A.lo[0], A.lo[1]}, hi[2] = {A.hi[0], A.hi[1]};
-      double x0[2] = {fmin(fmax(S.start[0], lo[0]), hi[0]), fmin(fmax(S.start[1], lo[1]), hi[1])};
-      bq_init(&bq, x0, lo, hi);
-      s_active = bq.status != BOBYQA_BAD_BOX && bq.want;
+      s_rejected = 0;
+      s_from_warm = 0;
+      s_cold[0] = fmin(fmax(S.start[0], A.lo[0]), A.hi[0]);
+      s_cold[1] = fmin(fmax(S.start[1], A.lo[1]), A.hi[1]);
+      s_active = 0;
+      if (A.pred) {
+        const float* pr = A.pred + 2 * (size_t)A.order[slot];
+        s_warm[0] = fmin(fmax(exp((double)pr[0]), A.lo[0]), A.hi[0]);
+        s_warm[1] = fmin(fmax(exp(-(double)pr[1]), A.lo[1]), A.hi[1]);
+        begin_search(PH_WARM);
+      }
+      if (!s_active) begin_search(PH_COLD);
     }
     __syncthreads();
     while (s_active) {
-      // request for the asked point
+      // request for the asked point (the cold start when checking a warm result)
       if (tid == 0) {
-        s_subject = bq.xask[0];
-        s_cell = bq.xask[1];
+        s_subject = s_phase == PH_CHECK ? s_cold[0] : bq.xask[0];
+        s_cell = s_phase == PH_CHECK ? s_cold[1] : bq.xask[1];
       }
       __syncthreads();
       const double subject = s_subject, cell = s_cell;
@@ -147,28 +180,57 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4)))
       if (tid == 0) {
         if (s_ok) s_have_warm = 1;
         const double v = s_value;
-        bool pass_over = false, found = false;
-        if (!isfinite(v)) {
-          pass_over = true;
-        } else {
-          bq_tell(&bq, v);
-          if (!bq.want) {
-            pass_over = true;
-            found = true;
-          }
-        }
-        if (pass_over) {
-          if (found || s_order == 1) {
+        if (s_phase == PH_CHECK) {
+          // keep the warm result unless the cold start point does better
+          if (isfinite(v) && s_wf <= v) {
             s_active = 0;
-            s_found = found ? 1 : 0;
-            s_failed = bq.status == BOBYQA_ROUNDOFF_LIMITED ? 1 : 0;
+            s_found = 1;
+            s_failed = s_wfailed;
+            s_from_warm = 1;
           } else {
-            s_order = 1;
-            const double lo[2] = {A.lo[0], A.lo[1]}, hi[2] = {A.hi[0], A.hi[1]};
-            double x0[2] = {fmin(fmax(S.start[0], lo[0]), hi[0]),
-                            fmin(fmax(S.start[1], lo[1]), hi[1])};
-            bq_init(&bq, x0, lo, hi);
-            s_active = bq.status != BOBYQA_BAD_BOX && bq.want;
+            s_rejected = 1;
+            s_order = (int)S.ord;
+            s_have_warm = 0;
+            begin_search(PH_COLD);
+          }
+        } else {
+          bool pass_over = false, found = false;
+          if (!isfinite(v)) {
+            pass_over = true;
+          } else {
+            bq_tell(&bq, v);
+            if (!bq.want) {
+              pass_over = true;
+              found = true;
+            }
+          }
+          if (pass_over) {
+            if (found || s_order == 1) {
+              if (s_phase == PH_WARM && found) {
+                s_wx[0] = bq.xbest_out[0];
+                s_wx[1] = bq.xbest_out[1];
+                s_wf = bq.fbest_out;
+                s_wfailed = bq.status == BOBYQA_ROUNDOFF_LIMITED ? 1 : 0;
+                s_phase = PH_CHECK;
+              } else if (s_phase == PH_WARM) {
+                s_rejected = 1;
+                s_order = (int)S.ord;
+                s_have_warm = 0;
+                begin_search(PH_COLD);
+              } else {
+                s_active = 0;
+                s_found = found ? 1 : 0;
+                s_failed = bq.status == BOBYQA_ROUNDOFF_LIMITED ? 1 : 0;
+              }
+            } else {
+              s_order = 1;
+              begin_search(s_phase);
+              if (!s_active && s_phase == PH_WARM) {
+                s_rejected = 1;
+                s_order = (int)S.ord;
+                begin_search(PH_COLD);
+              }
+            }
           }
         }
       }
@@ -176,13 +238,14 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4)))
     }
     if (tid == 0) {
       DevSearchOut o;
-      o.x[0] = bq.xbest_out[0];
-      o.x[1] = bq.xbest_out[1];
-      o.f = bq.fbest_out;
+      o.x[0] = s_from_warm ? s_wx[0] : bq.xbest_out[0];
+      o.x[1] = s_from_warm ? s_wx[1] : bq.xbest_out[1];
+      o.f = s_from_warm ? s_wf : bq.fbest_out;
       o.found = s_found;
       o.failed = s_failed;
       o.evals = s_evals;
       o.ord = s_order;
+      o.warm_rejected = s_rejected;
       A.out[A.order[slot]] = o;
     }
     __syncthreads();
@@ -207,7 +270,9 @@ template <int NB>
 static hipError_t launch_nb(const PersistentArgs& A, int waves, int grid, hipStream_t st) {
   switch (waves) {
     case 4: return launch_t<NB, 4>(A, grid, st);
-    case 16: return launch_t<NB, 16>(A, grid, st);
+    case 16:
+      if constexpr (NB <= 4) return launch_t<NB, 16>(A, grid, st);
+      [[fallthrough]];
     default: return launch_t<NB, 8>(A, grid, st);
   }
 }
@@ -230,7 +295,7 @@ hipError_t launch_persistent(const PersistentArgs& A, int waves, int grid, hipSt
 // Host side: build the search descriptors, order genes longest first, launch
 // once (the launch pair is captured in a hipGraph and replayed), read back.
 void run_stage_two_batched(Prepared& P, Resident& R, const nebula_hip_options_t& o,
-                           nebula_run_stats_t& st) {
+                           nebula_run_stats_t& st, const LearnedStart* learned) {
   const size_t G = P.kept.size(), nb = P.n_coef, k = P.k;
   const nebula_params_t& p = P.params;
   std::vector<DevSearch> hs;
@@ -271,13 +336,16 @@ void run_stage_two_batched(Prepared& P, Resident& R, const nebula_hip_options_t&
   int dev_cus = 0;
   NH_HIP(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, R.device));
   // tile shape: one gene per workgroup; wide workgroups when genes are few
+  // waves per gene: the tuned or requested size; otherwise 16 (a whole CU per
+  // gene) while genes are few per CU, when the slowest searches set the end
   int waves = 8;
   if (o.wg_size >= 1024)
     waves = 16;
   else if (o.wg_size > 0 && o.wg_size <= 256)
     waves = 4;
-  else if (o.wg_size == 0 && (int)n <= dev_cus)
+  else if (o.wg_size == 0 && (int)n <= 4 * dev_cus)
     waves = 16;
+  waves = effective_waves((int)nb, waves);
   const int per_cu = waves == 16 ? 1 : (waves == 8 ? 2 : 4);
   const int grid = (int)std::min<size_t>(n, (size_t)std::max(1, dev_cus * per_cu));
   DevBuf<DevSearch> dsearch;
@@ -316,6 +384,22 @@ void run_stage_two_batched(Prepared& P, Resident& R, const nebula_hip_options_t&
   A.hi[1] = p.max_cell;
   A.intercept = (uint32_t)P.intercept;
   A.counters = R.counters.p;
+  A.pred = nullptr;
+  A.warm_step = 0.0;
+  A.warm_xtol = 0.0;
+  DevBuf<float> dfeat, dpred;
+  if (learned) {
+    // MLP starts for the searched genes, in search order, on the same stream
+    std::vector<float> hf(n * 12);
+    for (size_t q = 0; q < n; ++q)
+      std::memcpy(&hf[q * 12], &P.features[(size_t)idx_of[q] * 12], 12 * sizeof(float));
+    dfeat.upload(hf.data(), hf.size(), R.stream);
+    dpred.alloc(n * 2);
+    NH_HIP(launch_mlp_predict(learned->params, dfeat.p, (int)n, dpred.p, R.stream));
+    A.pred = dpred.p;
+    A.warm_step = learned->step;
+    A.warm_xtol = learned->xtol_rel;
+  }
   // capture the queue reset + persistent launch once, replay it
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -347,6 +431,7 @@ void run_stage_two_batched(Prepared& P, Resident& R, const nebula_hip_options_t&
     pl.v[1] = ho[q].x[1];
     pl.objective = ho[q].f;
     pl.evaluations = ho[q].evals;
+    pl.warm_rejected = ho[q].warm_rejected != 0;
     st.evaluations += (uint64_t)ho[q].evals;
   }
 }

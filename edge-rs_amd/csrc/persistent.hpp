@@ -23,6 +23,7 @@ struct DevSearchOut {
   double x[2];
   double f;
   int found, failed, evals, ord;
+  int warm_rejected;  // learned start fell back to the cold search
 };
 
 struct PersistentArgs {
@@ -41,6 +42,9 @@ struct PersistentArgs {
   double lo[2], hi[2];
   uint32_t intercept;
   FitCounters* counters;
+  const float* pred;      // learned starts: n_search x (log sigma^2, log phi), or null
+  double warm_step;       // warm search's initial step relative to the predicted point
+  double warm_xtol;       // warm search's rhoend / rhobeg
 };
 
 size_t persistent_stride(int nb, int k, int waves);

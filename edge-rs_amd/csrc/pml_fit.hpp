@@ -102,6 +102,12 @@ struct Dims {
   static constexpr int NTRI = NB * (NB + 1) / 2;
 };
 
+// Occupancy target of the fit kernels (waves per SIMD): the f64 accumulators of
+// a sweep grow as p^2 / 2, so wide designs trade occupancy for no spills in
+// the sweep loop (128 / 168 / 256 VGPRs).
+template <int NB>
+constexpr int kWavesPerEu = NB <= 4 ? 4 : (NB <= 6 ? 3 : 2);
+
 __host__ __device__ inline uint32_t packed_row_width(uint32_t nb) {
   const uint32_t r = nb;  // nb - 1 columns + offset
   return r <= 1 ? 1u : (r <= 2 ? 2u : (r <= 4 ? 4u : 8u));

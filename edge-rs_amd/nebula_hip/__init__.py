@@ -25,6 +25,8 @@ _u32p = C.POINTER(C.c_uint32)
 
 ENGINE_PLAIN = 0
 ENGINE_BATCHED = 1
+ENGINE_LEARNED = 2
+WEIGHTS_PATH = os.path.join(HERE, "warmstart_weights.npy")
 
 
 # ---------------------------------------------------------------------------
@@ -86,7 +88,8 @@ class _Fit(C.Structure):
 class _Options(C.Structure):
     _fields_ = [("engine", C.c_int32), ("learned", C.c_int32),
                 ("mlp_weights", _fp), ("wg_size", C.c_int32),
-                ("device", C.c_int32), ("host_threads", C.c_int32)]
+                ("device", C.c_int32), ("host_threads", C.c_int32),
+                ("learned_step", C.c_double), ("learned_xtol", C.c_double)]
 
 
 class RunStats(C.Structure):
@@ -154,6 +157,18 @@ def lib():
         L.nh_bobyqa.restype = C.c_int
         L.nh_bobyqa.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_int, OBJECTIVE_FN,
                                 C.c_void_p, _dp, _dp, C.POINTER(C.c_int)]
+        L.nh_mlp_param_count.restype = C.c_size_t
+        L.nh_mlp_param_count.argtypes = []
+        L.nh_mlp_train.restype = C.c_int
+        L.nh_mlp_train.argtypes = [_fp, _fp, C.c_size_t, C.c_int32, C.c_int32, C.c_float,
+                                   C.c_uint64, C.c_int32, _fp, C.POINTER(C.c_float),
+                                   C.c_char_p, C.c_size_t]
+        L.nh_mlp_loss_grad.restype = C.c_int
+        L.nh_mlp_loss_grad.argtypes = [_fp, _fp, _fp, C.c_size_t, C.c_int32,
+                                       C.POINTER(C.c_float), _fp, C.c_char_p, C.c_size_t]
+        L.nh_mlp_predict.restype = C.c_int
+        L.nh_mlp_predict.argtypes = [_fp, _fp, C.c_size_t, C.c_int32, _fp, C.c_char_p,
+                                     C.c_size_t]
         L.nh_bobyqa_device_trace.restype = C.c_int
         L.nh_bobyqa_device_trace.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_int,
                                              C.POINTER(C.c_int), C.c_char_p, C.c_size_t]
@@ -229,7 +244,8 @@ def nebula_sparse_hip(counts: CompressedSparse, subject_id, design, n_coef: int,
                       offset=None, params: Optional[NebulaParams] = None,
                       engine: int = ENGINE_BATCHED, learned: bool = False,
                       mlp_weights=None, wg_size: int = 0, device: int = 0,
-                      host_threads: int = 0) -> NebulaFit:
+                      host_threads: int = 0, learned_step: float = 0.0,
+                      learned_xtol: float = 0.0) -> NebulaFit:
     """NEBULA with stages one to three on the GPU (edge-rs nebula_sparse_gpu)."""
     if counts.format != "csr":
         raise MalformedSparse("nebula_sparse needs gene-major counts in CSR form.")
@@ -261,6 +277,8 @@ def nebula_sparse_hip(counts: CompressedSparse, subject_id, design, n_coef: int,
     o.wg_size = wg_size
     o.device = device
     o.host_threads = host_threads
+    o.learned_step = learned_step
+    o.learned_xtol = learned_xtol
     fit = _Fit()
     st = RunStats()
     err = C.create_string_buffer(1024)
@@ -416,7 +434,8 @@ class Session:
     device (repeatable, timeable); finish() runs stage three."""
 
     def __init__(self, counts: CompressedSparse, subject_id, design, n_coef, offset=None,
-                 params: Optional[NebulaParams] = None, wg_size=0, device=0, host_threads=0):
+                 params: Optional[NebulaParams] = None, wg_size=0, device=0, host_threads=0,
+                 mlp_weights=None, learned_step=0.0, learned_xtol=0.0):
         L = lib()
         if not hasattr(L, "_session_bound"):
             L.nh_session_create.restype = C.c_int
@@ -436,6 +455,13 @@ class Session:
             L.nh_session_finish.argtypes = [C.c_void_p, C.POINTER(_Fit), _dp, C.c_char_p,
                                             C.c_size_t]
             L.nh_session_free.argtypes = [C.c_void_p]
+            L.nh_session_features.restype = C.c_int
+            L.nh_session_features.argtypes = [C.c_void_p, _fp]
+            L.nh_session_set_mlp.restype = C.c_int
+            L.nh_session_set_mlp.argtypes = [C.c_void_p, _fp, C.c_char_p, C.c_size_t]
+            L.nh_session_autotune.restype = C.c_int
+            L.nh_session_autotune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), _dp,
+                                              C.c_char_p, C.c_size_t]
             L._session_bound = True
         n_genes, n_cells = counts.shape
         self._keep = [np.ascontiguousarray(counts.indptr, dtype=np.uint32),
@@ -454,6 +480,12 @@ class Session:
         o.wg_size = wg_size
         o.device = device
         o.host_threads = host_threads
+        o.learned_step = learned_step
+        o.learned_xtol = learned_xtol
+        w = None
+        if mlp_weights is not None:
+            w = np.ascontiguousarray(mlp_weights, dtype=np.float32)
+            o.mlp_weights = w.ctypes.data_as(_fp)
         h = C.c_void_p()
         err = C.create_string_buffer(1024)
         rc = L.nh_session_create(n_genes, n_cells, ip.ctypes.data_as(_u32p),
@@ -471,6 +503,31 @@ class Session:
         lib().nh_session_stage_one(self._h, sg.ctypes.data_as(_dp),
                                    ords.ctypes.data_as(C.POINTER(C.c_int32)))
         return sg, ords
+
+    def features(self):
+        """The 12 warm-start features of every kept gene (kept x 12)."""
+        f = np.zeros((self.n_kept, 12), dtype=np.float32)
+        lib().nh_session_features(self._h, f.ctypes.data_as(_fp))
+        return f
+
+    def set_mlp(self, params):
+        w = np.ascontiguousarray(params, dtype=np.float32)
+        err = C.create_string_buffer(1024)
+        rc = lib().nh_session_set_mlp(self._h, w.ctypes.data_as(_fp), err, 1024)
+        if rc != 0:
+            _raise(rc, err)
+
+    def autotune(self, engine=ENGINE_BATCHED):
+        """Times stage two at 4, 8 and 16 waves per gene and keeps the fastest;
+        returns (threads per gene, {threads: ms})."""
+        ms = np.zeros(3)
+        pick = C.c_int32()
+        err = C.create_string_buffer(1024)
+        rc = lib().nh_session_autotune(self._h, engine, C.byref(pick), ms.ctypes.data_as(_dp),
+                                       err, 1024)
+        if rc != 0:
+            _raise(rc, err)
+        return pick.value, {256: ms[0], 512: ms[1], 1024: ms[2]}
 
     def stage_two(self, engine=ENGINE_BATCHED):
         mins = np.zeros((self.n_kept, 2))
@@ -519,3 +576,64 @@ class Session:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# Warm-start MLP (device kernels behind nh_mlp_*).
+def mlp_param_count():
+    return int(lib().nh_mlp_param_count())
+
+
+def _f(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(_fp)
+
+
+def mlp_train(features, targets, epochs=60, batch=256, lr=2e-3, seed=0, device=0):
+    """Trains the MLP on the device; returns (parameter blob, final loss)."""
+    f, fp = _f(features)
+    t, tp = _f(targets)
+    assert f.shape[1] == 12 and t.shape[1] == 2 and f.shape[0] == t.shape[0]
+    out = np.zeros(mlp_param_count(), dtype=np.float32)
+    loss = C.c_float()
+    err = C.create_string_buffer(1024)
+    rc = lib().nh_mlp_train(fp, tp, f.shape[0], epochs, batch, lr, seed, device,
+                            out.ctypes.data_as(_fp), C.byref(loss), err, 1024)
+    if rc != 0:
+        _raise(rc, err)
+    return out, loss.value
+
+
+def mlp_loss_grad(params, features, targets, device=0):
+    """Loss and gradient (network part of the blob) from the device kernels."""
+    w, wp = _f(params)
+    f, fp = _f(features)
+    t, tp = _f(targets)
+    g = np.zeros(mlp_param_count() - 28, dtype=np.float32)
+    loss = C.c_float()
+    err = C.create_string_buffer(1024)
+    rc = lib().nh_mlp_loss_grad(wp, fp, tp, f.shape[0], device, C.byref(loss),
+                                g.ctypes.data_as(_fp), err, 1024)
+    if rc != 0:
+        _raise(rc, err)
+    return loss.value, g
+
+
+def mlp_predict(params, features, device=0):
+    """BF16-MFMA inference: n x (log sigma^2, log phi)."""
+    w, wp = _f(params)
+    f, fp = _f(features)
+    out = np.zeros((f.shape[0], 2), dtype=np.float32)
+    err = C.create_string_buffer(1024)
+    rc = lib().nh_mlp_predict(wp, fp, f.shape[0], device, out.ctypes.data_as(_fp), err, 1024)
+    if rc != 0:
+        _raise(rc, err)
+    return out
+
+
+def load_weights(path=WEIGHTS_PATH):
+    """The trained warm-start parameters shipped with the package."""
+    w = np.load(path, allow_pickle=False).astype(np.float32)
+    if w.size != mlp_param_count():
+        raise ValueError(f"{path}: {w.size} parameters, expected {mlp_param_count()}")
+    return w

@@ -98,7 +98,9 @@ typedef struct {
 /* Which stage-two engine nebula_sparse_hip_ex runs. */
 enum {
   NH_ENGINE_PLAIN = 0,   /* host BOBYQA, one fit launch per search round */
-  NH_ENGINE_BATCHED = 1  /* persistent work-queue kernel, search on device */
+  NH_ENGINE_BATCHED = 1, /* persistent work-queue kernel, search on device */
+  NH_ENGINE_LEARNED = 2  /* batched, each search started from the MLP's
+                            prediction with the cold search as fallback */
 };
 
 /* Options beyond the reference's arguments (no counterpart in edge-rs). */
@@ -109,6 +111,9 @@ typedef struct {
   int32_t wg_size;       /* threads per fit workgroup, 0 = autotuned */
   int32_t device;        /* HIP device ordinal */
   int32_t host_threads;  /* 0 = hardware concurrency */
+  double learned_step;   /* warm search's initial BOBYQA step relative to the
+                            predicted point (0 = default 0.2) */
+  double learned_xtol;   /* warm search's rhoend / rhobeg (0 = default 1e-3) */
 } nebula_hip_options_t;
 
 /* Per-phase timings and counters of one call (EDGE_RS_GPU_TIMING's phases,
@@ -179,12 +184,24 @@ int nh_session_create(size_t n_genes, size_t n_cells, const uint32_t* indptr,
 size_t nh_session_genes(const nh_session_t* s);
 /* Stage one's variance components (kept x 2: sigma, gamma) and Laplace order. */
 int nh_session_stage_one(const nh_session_t* s, double* sigma_gamma, int32_t* ord);
+/* The 12 warm-start features of every kept gene (kept x 12 floats):
+ * log rate, positive fraction, log variance, log moment dispersion, 10/50/90 %
+ * quantiles and sd of the per-subject log rates, log nnz, log cells per
+ * subject, stage one's log sigma^2 and log phi. */
+int nh_session_features(const nh_session_t* s, float* out);
+/* Installs MLP parameters (nh_mlp_param_count floats) for NH_ENGINE_LEARNED. */
+int nh_session_set_mlp(nh_session_t* s, const float* params, char* err, size_t err_len);
 /* Runs stage two with engine NH_ENGINE_*.  minimisers: kept x 2 (subject,
  * cell), objectives: kept (the profile objective at the minimiser),
  * evaluations: kept; any may be NULL. */
 int nh_session_stage_two(nh_session_t* s, int32_t engine, double* minimisers,
                          double* objectives, int32_t* evaluations,
                          nebula_run_stats_t* stats, char* err, size_t err_len);
+/* Autotuner: runs stage two with engine once per workgroup size (256, 512,
+ * 1024 threads per gene, i.e. 4, 8, 16 waves), keeps the fastest for later
+ * calls of this session; ms (3 values) gets each size's stage-two time. */
+int nh_session_autotune(nh_session_t* s, int32_t engine, int32_t* wg_size, double* ms,
+                        char* err, size_t err_len);
 int nh_session_finish(nh_session_t* s, nebula_fit_t* out, double* t_stage_three,
                       char* err, size_t err_len);
 void nh_session_free(nh_session_t* s);
@@ -265,8 +282,34 @@ int nh_bobyqa_device_trace(int problem, int n, const double* x0,
                            double* trace, int max_trace, int* evals_out,
                            char* err, size_t err_len);
 
-/* Number of float parameters of the warm-start MLP (12 -> 64 -> 64 -> 64 -> 2). */
+/* ---------------------------------------------------------------------------
+ * Warm-start MLP (no counterpart in edge-rs): 12 features -> 64 -> 64 -> 64
+ * -> 2 (log sigma^2, log phi), GELU (tanh form).  Parameter blob layout:
+ * feature mean[12], feature sd[12], W1[64x12], b1[64], W2[64x64], b2[64],
+ * W3[64x64], b3[64], W4[2x64], b4[2], target mean[2], target sd[2]; W
+ * row-major (out x in).  The network predicts the correction to stage one's
+ * estimate (features 10 and 11): targets are standardised (log sigma^2, log
+ * phi) minus those features; predictions add them back.
+ * ------------------------------------------------------------------------- */
 size_t nh_mlp_param_count(void);
+/* Trains on the device (FP32 kernels, Adam, cosine learning-rate decay,
+ * minibatch `batch`, Huber loss with delta 1 on the standardised correction):
+ * features n x 12, targets n x 2 = (log sigma^2, log phi).  Standardisation is
+ * taken from the data; weights start Glorot-uniform from `seed`.  Writes the
+ * blob to params and the final mean loss over all rows to final_loss. */
+int nh_mlp_train(const float* features, const float* targets, size_t n, int32_t epochs,
+                 int32_t batch, float lr, uint64_t seed, int32_t device, float* params,
+                 float* final_loss, char* err, size_t err_len);
+/* Training loss (mean Huber, delta 1, over the standardised corrections) and
+ * its gradient with respect to the network part of the blob (W1..b4 in blob
+ * order), computed by the training kernels on the device. */
+int nh_mlp_loss_grad(const float* params, const float* features, const float* targets,
+                     size_t n, int32_t device, float* loss, float* grads, char* err,
+                     size_t err_len);
+/* Inference with BF16 weights and activations on the matrix cores (FP32
+ * accumulation): out n x 2 = (log sigma^2, log phi). */
+int nh_mlp_predict(const float* params, const float* features, size_t n, int32_t device,
+                   float* out, char* err, size_t err_len);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,171 @@
+"""The C ABI without a GPU: libnebula_hip.so loads and exports every function
+include/nebula_hip.h declares; the host BOBYQA stepper retraces nloptr point
+for point (src/numeric/bobyqa.rs:2947-2970 fixtures); input validation maps
+to the EdgeErrors variants before any device work (src/sc/nebula.rs:3303-3514,
+tests/e2e_nebula_gpu.rs:854-903)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import nebula_hip as nh
+from fixtures import dense_to_csr, golden8, read_matrix
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "nebula_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set()
+    for m in re.finditer(r"^\s*(?:int|void|size_t|double)\s*\**\s*(\w+)\s*\(", text, re.M):
+        names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_function():
+    names = declared_functions()
+    assert "nebula_sparse_hip" in names and "nh_resident_solve" in names
+    lib = C.CDLL(nh.LIB_PATH)
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+PROBLEMS = {
+    "rosen": (lambda x: 100.0 * (x[1] - x[0] * x[0]) ** 2 + (1.0 - x[0]) ** 2,
+              [-1.2, 1.0], [-2.0, -1.0], [2.0, 3.0], 4),
+    "rosen_edge": (lambda x: 100.0 * (x[1] - x[0] * x[0]) ** 2 + (1.0 - x[0]) ** 2,
+                   [0.4001, 0.2], [0.4, 0.2], [2.0, 2.0], 4),
+    "corner": (lambda x: (x[0] - 3.0) ** 2 + 10.0 * (x[1] + 1.0) ** 2 + x[0] * x[1],
+               [1.0, 1.0], [0.0, 0.0], [2.0, 2.0], 4),
+    "roundoff": (lambda x: (x[0] + x[1] - 1.0) ** 2, [0.0, 0.0], [-2.0, -2.0], [2.0, 2.0], -4),
+    "chain3": (lambda x: (x[0] - 1.0) ** 2 + 100.0 * (x[1] - x[0] * x[0]) ** 2 + (x[2] - x[1]) ** 2
+               + 0.5 * (x[2] * x[2]) ** 2, [0.5, 2.0, -1.0], [-3.0] * 3, [3.0] * 3, 4),
+}
+
+
+def ripple(x):
+    u = 10000.0 * x[0] + 30000.0 * x[1]
+    t = abs(u - 2.0 * np.floor((u + 1.0) / 2.0))
+    return (x[0] - 0.25) ** 2 + 2.0 * (x[1] + 0.125) ** 2 + 0.5 * x[0] * x[1] + 0.5 * (t * t)
+
+
+PROBLEMS["rescue"] = (ripple, [-1.0, 0.25], [-2.0, -2.0], [2.0, 2.0], 4)
+
+
+@pytest.mark.parametrize("tag", list(PROBLEMS))
+def test_host_bobyqa_retraces_nloptr(tag):
+    """Every point the host stepper asks for, and its status, equal nloptr's."""
+    f, x0, lo, hi, status = PROBLEMS[tag]
+    want = read_matrix(f"bobyqa_{tag}.csv")
+    trace = []
+
+    def obj(x):
+        v = float(f(x))
+        trace.append(list(x) + [v])
+        return v
+
+    x, fx, evals, st = nh.bobyqa(obj, x0, lo, hi)
+    assert st == status
+    got = np.array(trace)
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)
+    assert evals == len(want)
+
+
+def _golden_inputs():
+    counts, subject, design, offset = golden8()
+    ip, ix, dv = dense_to_csr(counts)
+    return nh.CompressedSparse(dv, ix, ip, counts.shape), subject, design, offset
+
+
+def test_reml_is_rejected():
+    cs, subject, design, offset = _golden_inputs()
+    with pytest.raises(nh.InvalidArgument):
+        nh.nebula_sparse_hip(cs, subject, design, 3, offset, nh.NebulaParams(reml=True))
+
+
+def test_more_than_eight_columns_rejected():
+    cs, subject, design, offset = _golden_inputs()
+    rng = np.random.default_rng(0)
+    wide = np.column_stack([design] + [rng.normal(size=design.shape[0]) for _ in range(6)])
+    with pytest.raises(nh.InvalidArgument):
+        nh.nebula_sparse_hip(cs, subject, wide, 9, offset)
+
+
+def test_split_subject_rejected():
+    cs, subject, design, offset = _golden_inputs()
+    s = subject.copy()
+    s[0], s[-1] = s[-1], s[0]
+    with pytest.raises(nh.SubjectsNotContiguous):
+        nh.nebula_sparse_hip(cs, s, design, 3, offset)
+
+
+def test_missing_intercept_rejected():
+    cs, subject, design, offset = _golden_inputs()
+    d = design.copy()
+    d[:, 0] = np.arange(d.shape[0], dtype=float)
+    with pytest.raises(nh.MissingIntercept):
+        nh.nebula_sparse_hip(cs, subject, d, 3, offset)
+
+
+def test_zero_column_rejected():
+    cs, subject, design, offset = _golden_inputs()
+    d = design.copy()
+    d[:, 2] = 0.0
+    with pytest.raises(nh.InvalidArgument):
+        nh.nebula_sparse_hip(cs, subject, d, 3, offset)
+
+
+def test_single_subject_rejected():
+    cs, subject, design, offset = _golden_inputs()
+    with pytest.raises(nh.TooFewSubjects):
+        nh.nebula_sparse_hip(cs, np.zeros_like(subject), design, 3, offset)
+
+
+def test_negative_count_rejected():
+    cs, subject, design, offset = _golden_inputs()
+    bad = nh.CompressedSparse(cs.data.copy(), cs.indices, cs.indptr, cs.shape)
+    bad.data[3] = -1.0
+    with pytest.raises(nh.InvalidArgument):
+        nh.nebula_sparse_hip(bad, subject, design, 3, offset)
+
+
+def test_malformed_csr_rejected():
+    cs, subject, design, offset = _golden_inputs()
+    ix = cs.indices.copy()
+    ix[0] = 10_000
+    with pytest.raises(nh.MalformedSparse):
+        nh.nebula_sparse_hip(nh.CompressedSparse(cs.data, ix, cs.indptr, cs.shape), subject,
+                             design, 3, offset)
+    with pytest.raises(nh.MalformedSparse):
+        nh.nebula_sparse_hip(nh.CompressedSparse(cs.data, cs.indices, cs.indptr, cs.shape,
+                                                 format="csc"), subject, design, 3, offset)
+
+
+def test_invalid_bounds_and_tolerances():
+    cs, subject, design, offset = _golden_inputs()
+    with pytest.raises(nh.InvalidBounds):
+        nh.nebula_sparse_hip(cs, subject, design, 3, offset, nh.NebulaParams(max=(1e-5, 1000.0)))
+    with pytest.raises(nh.MustBePositive):
+        nh.nebula_sparse_hip(cs, subject, design, 3, offset, nh.NebulaParams(min=(0.0, 1e-4)))
+    with pytest.raises(nh.MustBePositive):
+        nh.nebula_sparse_hip(cs, subject, design, 3, offset, nh.NebulaParams(eps=0.0))
+    with pytest.raises(nh.MustBePositive):
+        nh.nebula_sparse_hip(cs, subject, design, 3, -np.asarray(offset))
+
+
+def test_shape_and_length_mismatch():
+    cs, subject, design, offset = _golden_inputs()
+    with pytest.raises(nh.ShapeMismatch):
+        nh.nebula_sparse_hip(cs, subject, design[:-1], 3, offset)
+    with pytest.raises(nh.LengthMismatch):
+        nh.nebula_sparse_hip(cs, subject[:-1], design, 3, offset)
+
+
+def test_everything_filtered():
+    cs, subject, design, offset = _golden_inputs()
+    with pytest.raises(nh.NoGenesAfterFiltering):
+        nh.nebula_sparse_hip(cs, subject, design, 3, offset, nh.NebulaParams(cpc=1e9))
