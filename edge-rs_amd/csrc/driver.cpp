@@ -970,6 +970,12 @@ int nh_session_features(const nh_session_t* s, float* out) {
   return NH_OK;
 }
 
+int nh_session_warm_rejected(const nh_session_t* s, uint8_t* out) {
+  if (!s || !out) return NH_ERR_INVALID_ARGUMENT;
+  for (size_t i = 0; i < s->P.plans.size(); ++i) out[i] = s->P.plans[i].warm_rejected ? 1 : 0;
+  return NH_OK;
+}
+
 int nh_session_set_mlp(nh_session_t* s, const float* params, char* err, size_t err_len) {
   if (!s || !params) {
     put_err(err, err_len, "null argument");

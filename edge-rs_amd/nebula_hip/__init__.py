@@ -459,6 +459,8 @@ class Session:
             L.nh_session_features.argtypes = [C.c_void_p, _fp]
             L.nh_session_set_mlp.restype = C.c_int
             L.nh_session_set_mlp.argtypes = [C.c_void_p, _fp, C.c_char_p, C.c_size_t]
+            L.nh_session_warm_rejected.restype = C.c_int
+            L.nh_session_warm_rejected.argtypes = [C.c_void_p, C.POINTER(C.c_uint8)]
             L.nh_session_autotune.restype = C.c_int
             L.nh_session_autotune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), _dp,
                                               C.c_char_p, C.c_size_t]
@@ -541,7 +543,10 @@ class Session:
                                         err, 1024)
         if rc != 0:
             _raise(rc, err)
-        return dict(minimisers=mins, objectives=obj, evaluations=ev, stats=st.as_dict())
+        rej = np.zeros(self.n_kept, dtype=np.uint8)
+        lib().nh_session_warm_rejected(self._h, rej.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return dict(minimisers=mins, objectives=obj, evaluations=ev, stats=st.as_dict(),
+                    warm_rejected=rej.astype(bool))
 
     def finish(self):
         fit = _Fit()

@@ -189,6 +189,9 @@ int nh_session_stage_one(const nh_session_t* s, double* sigma_gamma, int32_t* or
  * quantiles and sd of the per-subject log rates, log nnz, log cells per
  * subject, stage one's log sigma^2 and log phi. */
 int nh_session_features(const nh_session_t* s, float* out);
+/* Per kept gene, 1 where the last NH_ENGINE_LEARNED run rejected the warm
+ * result (the cold start point did better) and ran the cold search. */
+int nh_session_warm_rejected(const nh_session_t* s, uint8_t* out);
 /* Installs MLP parameters (nh_mlp_param_count floats) for NH_ENGINE_LEARNED. */
 int nh_session_set_mlp(nh_session_t* s, const float* params, char* err, size_t err_len);
 /* Runs stage two with engine NH_ENGINE_*.  minimisers: kept x 2 (subject,
