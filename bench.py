@@ -143,8 +143,10 @@ def main():
     tune = None
     if not a.no_tune and a.engine != "plain":
         picked, ms = sess.autotune(eng)
-        tune = {"threads_per_gene": picked, "ms_by_threads": {str(k): round(v, 2)
-                                                              for k, v in ms.items()}}
+        # sizes that launch the same kernel as a smaller one are not rerun (NaN)
+        tune = {"threads_per_gene": picked,
+                "ms_by_threads": {str(k): (round(float(v), 2) if np.isfinite(v) else None)
+                                  for k, v in ms.items()}}
         log(f"autotune: {tune}")
     for i in range(a.warmup):
         r = sess.stage_two(eng)

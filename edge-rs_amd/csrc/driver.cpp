@@ -27,6 +27,7 @@
 #include "bobyqa.hpp"
 #include "driver.hpp"
 #include "host_common.hpp"
+#include "kernels.hpp"
 #include "resident.hpp"
 #include "stage_kernels.hpp"
 #include "stage_one.hpp"
@@ -720,6 +721,7 @@ void run_stage_two_plain(Prepared& P, Resident& R, int waves, nebula_run_stats_t
         P.plans[i].found = true;
         P.plans[i].v[0] = s.b.xbest_out[0];
         P.plans[i].v[1] = s.b.xbest_out[1];
+        P.plans[i].objective = s.b.fbest_out;
         P.plans[i].failed = s.b.status == BOBYQA_ROUNDOFF_LIMITED;
       }
       return;
@@ -1082,7 +1084,18 @@ int nh_session_autotune(nh_session_t* s, int32_t engine, int32_t* wg_size, doubl
   const int32_t sizes[3] = {256, 512, 1024};
   double best = INFINITY;
   int32_t pick = 0;
+  int seen[3] = {0, 0, 0};
   for (int c = 0; c < 3; ++c) {
+    // sizes that launch the same kernel as an earlier one are not rerun
+    const int w = engine == NH_ENGINE_PLAIN ? effective_waves((int)s->P.n_coef, sizes[c] / 64)
+                                            : persistent_waves((int)s->P.n_coef, sizes[c] / 64);
+    bool dup = false;
+    for (int d = 0; d < c; ++d) dup = dup || seen[d] == w;
+    seen[c] = w;
+    if (dup) {
+      if (ms) ms[c] = NAN;
+      continue;
+    }
     s->tuned_wg = sizes[c];
     nebula_run_stats_t st;
     const int rc = nh_session_stage_two(s, engine, nullptr, nullptr, nullptr, &st, err, err_len);

@@ -47,9 +47,22 @@ inline int effective_waves(int nb, int waves) {
   if (waves >= 16) return nb <= 4 ? 16 : 8;
   return waves <= 4 ? 4 : 8;
 }
+
+// The same for the persistent stage-two kernel, which is built for 4 and 16
+// waves per gene when p <= 4 (an 8-wave request runs with 16): its 8-wave,
+// 128-VGPR instance returned objective values that disagree with the f64
+// oracle, while the same source at 4 or 16 waves, or at 8 waves with a
+// 256-VGPR budget, agrees to 1e-10.
+inline int persistent_waves(int nb, int waves) {
+  const int w = effective_waves(nb, waves);
+  return (nb <= 4 && w == 8) ? 16 : w;
+}
 // Largest per-workgroup work area placed in LDS (bytes); larger ones live in
 // global memory.
-constexpr size_t kWorkLdsBytes = 48 * 1024;
+#ifndef NH_WORK_LDS_BYTES
+#define NH_WORK_LDS_BYTES (48 * 1024)
+#endif
+constexpr size_t kWorkLdsBytes = NH_WORK_LDS_BYTES;
 hipError_t launch_batch(const BatchLaunch& L, hipStream_t st);
 hipError_t launch_bobyqa_trace(int problem, int n, const double* x0, const double* lo,
                                const double* hi, double* trace, int max_trace, int* result,

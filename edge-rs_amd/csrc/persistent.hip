@@ -89,10 +89,19 @@ void stage_two_kernel(PersistentArgs A) {
   auto begin_search = [&](int phase) {
     s_phase = phase;
     if (phase == PH_WARM) {
+      // The initial step is a fraction of the predicted value, widened to
+      // twice the distance to stage one's estimate where the two disagree
+      // (a prediction far from stage one is less certain).  The final
+      // precision rhoend = warm_xtol * warm_step * x stays the same whatever
+      // the step: xtol is scaled down when the step is widened.
       double step[2];
-      for (int i = 0; i < 2; ++i)
-        step[i] = fmin(fmax(A.warm_step * s_warm[i], 1e-3 * lo[i]), 0.45 * (hi[i] - lo[i]));
-      bq_init_step(&bq, s_warm, lo, hi, step, A.warm_xtol);
+      for (int i = 0; i < 2; ++i) {
+        const double base = A.warm_step * s_warm[i];
+        step[i] = fmax(base, 2.0 * fabs(s_warm[i] - s_cold[i]));
+        step[i] = fmin(fmax(step[i], 1e-3 * lo[i]), 0.45 * (hi[i] - lo[i]));
+      }
+      const double xtol = A.warm_xtol * fmin(1.0, A.warm_step * s_warm[0] / step[0]);
+      bq_init_step(&bq, s_warm, lo, hi, step, xtol);
     } else {
       bq_init(&bq, s_cold, lo, hi);
     }
@@ -268,12 +277,10 @@ static hipError_t launch_t(const PersistentArgs& A, int grid, hipStream_t st) {
 
 template <int NB>
 static hipError_t launch_nb(const PersistentArgs& A, int waves, int grid, hipStream_t st) {
-  switch (waves) {
-    case 4: return launch_t<NB, 4>(A, grid, st);
-    case 16:
-      if constexpr (NB <= 4) return launch_t<NB, 16>(A, grid, st);
-      [[fallthrough]];
-    default: return launch_t<NB, 8>(A, grid, st);
+  if constexpr (NB <= 4) {
+    return waves == 4 ? launch_t<NB, 4>(A, grid, st) : launch_t<NB, 16>(A, grid, st);
+  } else {
+    return waves == 4 ? launch_t<NB, 4>(A, grid, st) : launch_t<NB, 8>(A, grid, st);
   }
 }
 
@@ -345,7 +352,7 @@ void run_stage_two_batched(Prepared& P, Resident& R, const nebula_hip_options_t&
     waves = 4;
   else if (o.wg_size == 0 && (int)n <= 4 * dev_cus)
     waves = 16;
-  waves = effective_waves((int)nb, waves);
+  waves = persistent_waves((int)nb, waves);
   const int per_cu = waves == 16 ? 1 : (waves == 8 ? 2 : 4);
   const int grid = (int)std::min<size_t>(n, (size_t)std::max(1, dev_cus * per_cu));
   DevBuf<DevSearch> dsearch;

@@ -276,27 +276,31 @@ __device__ __forceinline__ float drcp<float>(float x) { return __frcp_rn(x); }
 template <>
 __device__ __forceinline__ double drcp<double>(double x) { return 1.0 / x; }
 
+typedef float gfloat4 __attribute__((ext_vector_type(4)));
+typedef float gfloat2 __attribute__((ext_vector_type(2)));
+typedef double gdouble2 __attribute__((ext_vector_type(2)));
+
 // One packed row of R values (16-byte vector loads where the row allows).
-template <class CT, int R>
-__device__ __forceinline__ void load_row(const CT* __restrict__ base, uint32_t cell, CT* out) {
-  const CT* p = base + (size_t)cell * R;
+template <class CT, int R, class P>
+__device__ __forceinline__ void load_row(P base, uint32_t cell, CT* out) {
+  const auto p = base + (size_t)cell * R;
   if constexpr (sizeof(CT) == 4 && R >= 4) {
 #pragma unroll
     for (int h = 0; h < R / 4; ++h) {
-      const float4 v = reinterpret_cast<const float4*>(p)[h];
+      const gfloat4 v = reinterpret_cast<__attribute__((address_space(1))) const gfloat4*>(p)[h];
       out[4 * h] = v.x;
       out[4 * h + 1] = v.y;
       out[4 * h + 2] = v.z;
       out[4 * h + 3] = v.w;
     }
   } else if constexpr (sizeof(CT) == 4 && R == 2) {
-    const float2 v = *reinterpret_cast<const float2*>(p);
+    const gfloat2 v = *reinterpret_cast<__attribute__((address_space(1))) const gfloat2*>(p);
     out[0] = v.x;
     out[1] = v.y;
   } else if constexpr (sizeof(CT) == 8 && R >= 2) {
 #pragma unroll
     for (int h = 0; h < R / 2; ++h) {
-      const double2 v = reinterpret_cast<const double2*>(p)[h];
+      const gdouble2 v = reinterpret_cast<__attribute__((address_space(1))) const gdouble2*>(p)[h];
       out[2 * h] = v.x;
       out[2 * h + 1] = v.y;
     }
@@ -321,6 +325,24 @@ struct Lds<float> {
 template <>
 struct Lds<double> {
   using ptr = lds_double*;
+};
+
+// Global-memory views of the inputs: loads through them are global_load
+// instructions (counted on vmcnt only and returned in order), so a wait for
+// one tile's rows leaves the next CSR window's loads in flight.  Generic
+// pointers would compile to flat loads, which every wait must drain.
+typedef __attribute__((address_space(1))) const float gconst_float;
+typedef __attribute__((address_space(1))) const double gconst_double;
+typedef __attribute__((address_space(1))) const uint32_t gconst_u32;
+template <class CT>
+struct Glb;
+template <>
+struct Glb<float> {
+  using ptr = gconst_float*;
+};
+template <>
+struct Glb<double> {
+  using ptr = gconst_double*;
 };
 
 // Raw hardware transcendentals for the FP32 sweeps: v_exp_f32 / v_log_f32
@@ -352,6 +374,9 @@ __device__ __forceinline__ void sweep_wave(const CT* __restrict__ rows,
   using D = Dims<NB>;
   constexpr int Q = D::Q, R = D::R, NS = D::NS, NSL = D::NSL, NG = D::NG;
   constexpr int U = sizeof(CT) == 4 ? kTileF32 : kTileF64;
+  const typename Glb<CT>::ptr grows = (typename Glb<CT>::ptr)rows;
+  const typename Glb<CT>::ptr ggy = (typename Glb<CT>::ptr)gy;
+  gconst_u32* const ggcell = (gconst_u32*)gcell;
   constexpr int TILE = kWave * U;
   constexpr int NSV = MODE == SW_LAPLACE ? 4 : NS;
   constexpr int NGV = MODE == SW_LAPLACE ? 0 : NG;
@@ -365,15 +390,21 @@ __device__ __forceinline__ void sweep_wave(const CT* __restrict__ rows,
 #pragma unroll
   for (int v = 0; v < (NGV > 0 ? NGV : 1); ++v) ag[v] = 0.0;
   if (cb < ce) {
-    // window of the next TILE entries of the gene's CSR row
+    // window of the next TILE entries of the gene's CSR row.  Every load is
+    // issued unconditionally at an in-bounds index (the last entry when past
+    // the row; buffers hold at least one entry) and masked afterwards, so the
+    // compiler can count outstanding loads and wait only for the ones needed.
+    const uint64_t e_last = g1 > 0 ? g1 - 1 : 0;
     uint32_t wc[U];
     CT wy[U];
 #pragma unroll
     for (int j = 0; j < U; ++j) {
       const uint64_t e = ip + (uint64_t)(lane + kWave * j);
-      const bool ok = e < g1;
-      wc[j] = ok ? gcell[e] : 0xffffffffu;
-      wy[j] = ok ? gy[e] : (CT)0;
+      const uint64_t ec = e < g1 ? e : e_last;
+      const uint32_t c = ggcell[ec];
+      const CT y = ggy[ec];
+      wc[j] = e < g1 ? c : 0xffffffffu;
+      wy[j] = y;
     }
     for (; s < k && fid[s] < ce; ++s) {
       const uint32_t lo = max(fid[s], cb), hi = min(fid[s + 1], ce);
@@ -388,12 +419,7 @@ __device__ __forceinline__ void sweep_wave(const CT* __restrict__ rows,
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const uint32_t cell = base + (uint32_t)(lane + kWave * u);
-          if (cell < tend) {
-            load_row<CT, R>(rows, cell, xr[u]);
-          } else {
-#pragma unroll
-            for (int r = 0; r < R; ++r) xr[u][r] = (CT)0;
-          }
+          load_row<CT, R>(grows, cell < tend ? cell : tend - 1, xr[u]);
         }
         // the window's entries inside [base, tend) form a prefix of it
         uint64_t m = 0;
@@ -407,9 +433,11 @@ __device__ __forceinline__ void sweep_wave(const CT* __restrict__ rows,
 #pragma unroll
         for (int j = 0; j < U; ++j) {
           const uint64_t e = ip + (uint64_t)(lane + kWave * j);
-          const bool ok = e < g1;
-          wc[j] = ok ? gcell[e] : 0xffffffffu;
-          wy[j] = ok ? gy[e] : (CT)0;
+          const uint64_t ec = e < g1 ? e : e_last;
+          const uint32_t c = ggcell[ec];
+          const CT y = ggy[ec];
+          wc[j] = e < g1 ? c : 0xffffffffu;
+          wy[j] = y;
         }
         __builtin_amdgcn_wave_barrier();
         CT yv[U];
