@@ -23,6 +23,16 @@ sys.path.insert(0, os.path.join(ROOT, "edge-rs_amd"))
 import numpy as np  # noqa: E402
 
 METRIC = "genes/sec NEBULA-HL stage 2, learned+batched vs plain HIP, |Δloglik|<1e-6 rel"
+# workload name -> (synth.config index, genes, cells); the default is the
+# headline 500 x 100k, p = 4 shape; the others are the remaining shapes of the
+# benchmark set, the last one at one GPU's share (1/8) of 20,000 genes
+WORKLOADS = {
+    "nebula_hl_stage_two_500g_100kc_20s_p4": (2, 500, 100_000),
+    "nebula_hl_stage_two_200g_5kc_10s_p2": (1, 200, 5_000),
+    "nebula_hl_stage_two_2000g_200kc_40s_p8": (3, 2000, 200_000),
+    "nebula_hl_stage_two_5000g_100kc_30s_p6": (4, 5000, 100_000),
+    "nebula_hl_stage_two_2500g_500kc_100s_p6": (5, 2500, 500_000),
+}
 WORKLOAD = "nebula_hl_stage_two_500g_100kc_20s_p4"
 HBM_PEAK_GBS = 8000.0
 TRAFFIC = os.path.join(ROOT, "profiles", "stage_two_kernel_traffic.json")
@@ -90,8 +100,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--genes", type=int, default=500)
-    ap.add_argument("--cells", type=int, default=100_000)
+    ap.add_argument("--workload", choices=list(WORKLOADS), default=WORKLOAD)
+    ap.add_argument("--genes", type=int, default=None, help="override the workload's genes")
+    ap.add_argument("--cells", type=int, default=None, help="override the workload's cells")
     ap.add_argument("--engine", choices=["learned", "batched", "plain"], default="learned",
                     help="engine timed for `value` (only learned runs under rocprofv3 "
                          "when profiling one kernel)")
@@ -126,7 +137,8 @@ def main():
     engines = {"learned": nh.ENGINE_LEARNED, "batched": nh.ENGINE_BATCHED,
                "plain": nh.ENGINE_PLAIN}
     t0 = time.time()
-    w = synth.config(2, n_genes=a.genes, n_cells=a.cells)
+    cfg, genes, cells = WORKLOADS[a.workload]
+    w = synth.config(cfg, n_genes=a.genes or genes, n_cells=a.cells or cells)
     nnz_frac = np.diff(w.indptr.astype(np.int64)) / w.n_cells
     log(f"workload: {w.n_genes} genes x {w.n_cells} cells, k={len(w.truth['sizes'])}, "
         f"p={w.n_coef}, nnz {w.nnz} (per gene {nnz_frac.min():.3f}..{nnz_frac.max():.3f}), "
@@ -190,7 +202,7 @@ def main():
     achieved = bytes_alg / launches / kern_s / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(WORKLOAD, a.engine),
+            "traffic": measured_traffic(a.workload, a.engine),
             "kernel": "pml_batch_kernel" if a.engine == "plain" else "stage_two_kernel",
             "kernel_ms_per_launch": round(1e3 * kern_s, 3),
             "algorithmic_bytes_per_launch": int(bytes_alg / launches)}
@@ -246,9 +258,9 @@ def main():
     out = {"metric": METRIC, "value": round(value, 2), "unit": "genes/sec", "n_gpus": 1,
            "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
-           "data": "synthetic (seeded NB-gamma mixed model: Dirichlet(2) subject sizes, "
-                   "beta0 ~ N(-3, 1.5), offset = lib / mean(lib))",
-           "config": {"workload": WORKLOAD, "genes": w.n_genes, "cells": w.n_cells,
+           "data": "synthetic (seeded NB-gamma mixed model, offset = lib / mean(lib); "
+                   f"generator of shape {cfg})",
+           "config": {"workload": a.workload, "genes": w.n_genes, "cells": w.n_cells,
                       "subjects": int(len(w.truth['sizes'])), "n_coef": w.n_coef,
                       "genes_searched": searched, "engine": a.engine},
            "roofline": roof, "cpu_baseline": cpu}
