@@ -18,6 +18,12 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+
 __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __ballot(p); }
 
 __device__ __forceinline__ int popc64(uint64_t m) { return __popcll(m); }

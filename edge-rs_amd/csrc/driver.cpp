@@ -765,7 +765,7 @@ void run_stage_two_plain(Prepared& P, Resident& R, int waves, nebula_run_stats_t
       who.push_back(i);
     }
     if (who.empty()) break;
-    R.solve(req, bi, &lwi, p.eps, 0, waves, out, ob, olw, nullptr);
+    R.solve(req, bi, &lwi, p.eps, 0, waves, out, ob, olw, nullptr, true);
     ++st.rounds;
     for (size_t q = 0; q < who.size(); ++q) {
       const size_t i = who[q];

@@ -385,6 +385,7 @@ void run_stage_two_batched(Prepared& P, Resident& R, const nebula_hip_options_t&
   A.knobs.noise_rel = 1e-8;
   A.knobs.max_iter = 50;
   A.knobs.max_backtrack = 10;
+  A.knobs.decrement_stop = 1;
   A.lo[0] = p.min_subject;
   A.lo[1] = p.min_cell;
   A.hi[0] = p.max_subject;

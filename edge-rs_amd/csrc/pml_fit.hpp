@@ -69,6 +69,11 @@ struct FitKnobs {
   double noise_rel;  // f32 resolution floor relative to |loglik|
   int max_iter;      // Newton budget (50)
   int max_backtrack; // 10
+  // FP32 Newton: take a step whose Newton decrement (predicted
+  // log-likelihood gain) is below the resolution floor without the sweep
+  // that would measure it; the FP64 finish sweeps that point next.  0 keeps
+  // the measured rule.
+  int decrement_stop;
 };
 
 // Counters every fit adds to (device-wide, relaxed atomics).
@@ -761,6 +766,34 @@ struct Fitter {
     __syncthreads();
   }
 
+  // Newton decrement of the step newton_step(sd) just wrote: 0.5 g'D with g
+  // the gradient at side sd and D the full step, the log-likelihood gain the
+  // quadratic model predicts (the step solves I D = g, I positive definite).
+  // NaN when a coordinate moves by more than 0.1, where the model is not
+  // trusted.
+  __device__ __noinline__ double decrement(int sd) {
+    const double* sub = side(sd);
+    if (wave == 0) {
+      double acc = 0.0, big = 0.0;
+      for (int s = lane; s < k; s += kWave) {
+        acc += sub[s * SUB + 0] * wk.step_lw[s];
+        big = fmax(big, fabs(wk.step_lw[s]));
+      }
+      acc = wave_sum(acc);
+      big = wave_max(big);
+      if (lane == 0) {
+        double g = acc;
+        for (int j = 0; j < NB; ++j) {
+          g += sh.db[sd][j] * sh.step_beta[j];
+          big = fmax(big, fabs(sh.step_beta[j]));
+        }
+        sh.scal[4] = big <= 0.1 ? 0.5 * g : NAN;
+      }
+    }
+    __syncthreads();
+    return sh.scal[4];
+  }
+
   // Per-coordinate damping of the trial point (pml.rs:1729-1755).
   __device__ __noinline__ void damp(double min_step) {
     const double cut = 40.0;
@@ -827,6 +860,17 @@ struct Fitter {
       ++step;
       newton_step(cur);
       const int tr = 1 - cur;
+      if (noise_rel > 0.0 && kn.decrement_stop) {
+        // f32: a step predicted to gain less than the resolution floor is
+        // taken without the sweep that would measure it; the f64 finish
+        // sweeps the stepped point next.
+        const double dec = decrement(cur);
+        if (dec >= 0.0 && dec <= fmax(kn.eps, noise_rel * fabs(ll))) {
+          accept();
+          ll_prev = ll;
+          break;
+        }
+      }
       ll_prev = ll;
       sweep<CT, SW_FULL>(sh.new_beta, wk.new_lw);
       finalize(tr, wk.new_lw);

@@ -118,7 +118,8 @@ FitCounters Resident::read_counters() {
 void Resident::solve(const std::vector<DevRequest>& r, const std::vector<double>& bi,
                      const std::vector<double>* lwi, double eps, int mode, int waves,
                      std::vector<DevReply>& out, std::vector<double>& ob,
-                     std::vector<double>& olw, std::vector<double>* oinfo) {
+                     std::vector<double>& olw, std::vector<double>* oinfo,
+                     bool decrement_stop) {
   const size_t n = r.size();
   out.resize(n);
   ob.resize(n * nb);
@@ -154,6 +155,7 @@ void Resident::solve(const std::vector<DevRequest>& r, const std::vector<double>
   L.knobs.noise_rel = 1e-8;
   L.knobs.max_iter = 50;
   L.knobs.max_backtrack = 10;
+  L.knobs.decrement_stop = decrement_stop ? 1 : 0;
   L.mode = mode;
   L.waves = waves;
   L.n_req = (int)n;

@@ -73,11 +73,13 @@ struct Resident {
               const uint64_t* subject_start, size_t n_genes, const uint64_t* gene_ptr,
               const uint32_t* cells, const double* counts, int device);
   DeviceData view() const;
-  // Runs n requests, replies filled on the host.
+  // Runs n requests, replies filled on the host.  decrement_stop: mode 0
+  // fits stop on the Newton decrement (stage-two evaluations).
   void solve(const std::vector<DevRequest>& r, const std::vector<double>& beta_init,
              const std::vector<double>* lw_init, double eps, int mode, int waves,
              std::vector<DevReply>& out, std::vector<double>& out_beta,
-             std::vector<double>& out_lw, std::vector<double>* out_info);
+             std::vector<double>& out_lw, std::vector<double>* out_info,
+             bool decrement_stop = false);
   FitCounters read_counters();
   void reset_counters();
 };
