@@ -83,7 +83,10 @@ struct FitCounters {
   unsigned long long cells_f64, nnz_f64;  // summed over FP64 sweeps
 };
 
-enum SweepMode { SW_FULL = 0, SW_LAPLACE = 2 };
+// SW_FINISH: SW_FULL plus, per subject, the eta-derivatives of the weight
+// sum (sum of d pw / d eta, and that times each stored column), from which
+// the FP64 finish moves the log-determinant to the stepped point.
+enum SweepMode { SW_FULL = 0, SW_FINISH = 1, SW_LAPLACE = 2 };
 
 // Cells per lane in one tile of a sweep (tile = 64 x U cells).
 #ifndef NH_TILE_F32
@@ -101,7 +104,8 @@ struct Dims {
   static constexpr int R = Q + 1 <= 1 ? 1 : (Q + 1 <= 2 ? 2 : (Q + 1 <= 4 ? 4 : 8));
   static constexpr int QT = Q * (Q + 1) / 2;
   static constexpr int NS = 2 + Q;                // per subject: resid, weight, first[Q]
-  static constexpr int NSL = NS > 4 ? NS : 4;     // slot width (Laplace keeps 4)
+  static constexpr int NS3 = NS + 1 + Q;          // SW_FINISH: + dweight, dweight x[Q]
+  static constexpr int NSL = NS3 > 4 ? NS3 : 4;   // slot width (Laplace keeps 4)
   static constexpr int NG = 1 + Q + QT;           // per gene: llp, score[Q], second[QT]
   static constexpr int SUB = 3 + NB;              // dw, vw, w, vwb[NB]
   static constexpr int NTRI = NB * (NB + 1) / 2;
@@ -383,7 +387,8 @@ __device__ __forceinline__ void sweep_wave(const CT* __restrict__ rows,
   const typename Glb<CT>::ptr ggy = (typename Glb<CT>::ptr)gy;
   gconst_u32* const ggcell = (gconst_u32*)gcell;
   constexpr int TILE = kWave * U;
-  constexpr int NSV = MODE == SW_LAPLACE ? 4 : NS;
+  constexpr int NS3 = D::NS3;
+  constexpr int NSV = MODE == SW_LAPLACE ? 4 : (MODE == SW_FINISH ? NS3 : NS);
   constexpr int NGV = MODE == SW_LAPLACE ? 0 : NG;
   constexpr int QA = Q > 0 ? Q : 1;
   CT bq[QA];
@@ -486,6 +491,13 @@ __device__ __forceinline__ void sweep_wave(const CT* __restrict__ rows,
             double xd[QA];
 #pragma unroll
             for (int q = 0; q < Q; ++q) xd[q] = (double)xr[u][q];
+            if constexpr (MODE == SW_FINISH) {
+              // d pw / d eta = pw (gamma - e) / t
+              const double dpw = pw * (((double)g - (double)ex) * (double)inv);
+              as[NS] += dpw;
+#pragma unroll
+              for (int q = 0; q < Q; ++q) as[NS + 1 + q] += dpw * xd[q];
+            }
 #pragma unroll
             for (int a = 0; a < Q; ++a) {
               const double pa = pw * xd[a];
@@ -597,7 +609,7 @@ struct Fitter {
   // leaving per-subject totals in wk.tsub and per-gene totals in wk.tgene.
   template <class CT, int MODE>
   __device__ __noinline__ void sweep(const double* bsrc, const double* lwsrc) {
-    constexpr int NSV = MODE == SW_LAPLACE ? 4 : NS;
+    constexpr int NSV = MODE == SW_LAPLACE ? 4 : (MODE == SW_FINISH ? D::NS3 : NS);
     constexpr int NGV = MODE == SW_LAPLACE ? 0 : NG;
     const CT* rows = sizeof(CT) == 4 ? (const CT*)dd.row32 : (const CT*)dd.row64;
     const CT* gy = sizeof(CT) == 4 ? (const CT*)dd.gy32 : (const CT*)dd.gy64;
@@ -904,18 +916,59 @@ struct Fitter {
     return cur;
   }
 
+  // sum_s log |vw_s| at the point newton_step(sd) stepped to, from side sd's
+  // vw and an SW_FINISH sweep's eta-derivatives of the weight sums.
+  __device__ __noinline__ double stepped_log_det(int sd) {
+    const double* sub = side(sd);
+    if (wave == 0) {
+      const double acc = subject_sum([&](int s) {
+        const double* t = wk.tsub + (size_t)s * NSL;
+        double dw = t[D::NS] * (wk.step_lw[s] + sh.step_beta[icpt]);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) dw += t[D::NS + 1 + q] * sh.step_beta[qcol(q, icpt)];
+        const double w1 = exp(wk.lw[s] + wk.step_lw[s]);
+        const double vw = sub[s * SUB + 1] + gamma * dw + lambda * (w1 - sub[s * SUB + 2]);
+        return log(fabs(vw));
+      });
+      if (lane == 0) sh.scal[5] = acc;
+    }
+    __syncthreads();
+    return sh.scal[5];
+  }
+
   // FP64 finish from the current point (newton_finish, pml.rs:807-1090):
   // Newton steps without damping, log-likelihood and log-determinant at the
   // stepped point.  Returns false when a step worsens the objective beyond
   // rounding or leaves it not finite, or the budget runs out.
   __device__ bool finish(const FitKnobs& kn, FitResult& out) {
     int cur = 0;
-    sweep<double, SW_FULL>(sh.beta, wk.lw);
+    if (kn.decrement_stop)
+      sweep<double, SW_FINISH>(sh.beta, wk.lw);
+    else
+      sweep<double, SW_FULL>(sh.beta, wk.lw);
     finalize(cur, wk.lw);
     for (int it = 1; it <= kn.max_iter; ++it) {
       const double ll_prev = sh.ll[cur];
       newton_step(cur);
       const int tr = 1 - cur;
+      if (kn.decrement_stop && it == 1) {
+        // A first step predicted to gain at most eps ends the fit as the
+        // measured rule below would after sweeping the stepped point: the
+        // log-likelihood there is ll + decrement (the quadratic model, exact
+        // to third order in the step) and each subject's vw moves to first
+        // order along d eta = step_lw + x . step_beta.
+        const double dec = decrement(cur);
+        if (dec >= 0.0 && dec <= kn.eps) {
+          const double ld = stepped_log_det(cur);
+          accept();
+          out.ll = ll_prev + dec;
+          out.ll_prev = ll_prev;
+          out.log_det = ld;
+          out.finish_steps = it;
+          out.second = 0.0;
+          return true;  // the information at the start point stays in vb2
+        }
+      }
       sweep<double, SW_FULL>(sh.new_beta, wk.new_lw);
       finalize(tr, wk.new_lw);
       const double ll = sh.ll[tr];
