@@ -178,6 +178,7 @@ struct FitLds {
   int perm[NB];
   double ll[2];
   double scal[8];
+  double f64tab[82];  // tables of the FP64 sweep's exp and log (Fitter::setup)
   uint64_t ip0[WAVES];
   uint32_t s0[WAVES];
   float ysc[WAVES][kWave * 4];  // count staging: 256 floats or 128 doubles per wave
@@ -356,13 +357,84 @@ struct Glb<double> {
 
 // Raw hardware transcendentals for the FP32 sweeps: v_exp_f32 / v_log_f32
 // (base 2) and v_rcp_f32, about 1 ulp; the arguments here are normal numbers
-// (t = e + gamma >= gamma > 0).
-__device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504f); }
-__device__ __forceinline__ float fast_log(float x) { return __builtin_amdgcn_logf(x) * 0.693147181f; }
+// (t = e + gamma >= gamma > 0).  The table argument is unused in FP32.
+__device__ __forceinline__ float fast_exp(float x, const lds_double*) {
+  return __builtin_amdgcn_exp2f(x * 1.44269504f);
+}
+__device__ __forceinline__ float fast_log(float x, const lds_double*) {
+  return __builtin_amdgcn_logf(x) * 0.693147181f;
+}
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ double fast_exp(double x) { return exp(x); }
-__device__ __forceinline__ double fast_log(double x) { return log(x); }
-__device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; }
+
+// FP64 exp, log and reciprocal of the FP64 sweeps, with short dependency
+// chains and no special-case branches (the sweep's arguments are finite;
+// overflow gives inf, underflow 0, NaN stays NaN).  Tables in LDS:
+//   [0, 16)   2^(j/16)
+//   [16, 49)  1 / c_j,  c_j = 1 + j/32, j = 0..32
+//   [49, 82)  log c_j (0 for j = 32, whose binade is moved up by one)
+constexpr int kTabExp = 0, kTabInvc = 16, kTabLogc = 49;
+constexpr double kLn2Hi = 6.93147180369123816490e-01;  // 32 significant bits
+constexpr double kLn2Lo = 1.90821492927058770002e-10;
+
+// exp(x) = 2^(i/16) e^r, i = rint(16 x / ln 2), |r| <= ln 2 / 32; e^r - 1 by
+// its Taylor series to r^7 (truncation < 2e-18 relative); <= 1 ulp.
+__device__ __forceinline__ double fast_exp(double x, const lds_double* tab) {
+  const double kd = __builtin_rint(x * 23.083120654223414);  // 16 / ln 2
+  double r = __builtin_fma(-kd, kLn2Hi * 0.0625, x);
+  r = __builtin_fma(-kd, kLn2Lo * 0.0625, r);
+  const int i = (int)kd;  // saturates for |x| beyond the double range of exp
+  const double s = tab[kTabExp + (i & 15)];
+  double q = __builtin_fma(r, 1.0 / 5040, 1.0 / 720);
+  q = __builtin_fma(r, q, 1.0 / 120);
+  q = __builtin_fma(r, q, 1.0 / 24);
+  q = __builtin_fma(r, q, 1.0 / 6);
+  q = __builtin_fma(r, q, 0.5);
+  const double em1 = __builtin_fma(r * r, q, r);
+  return __builtin_ldexp(__builtin_fma(s, em1, s), i >> 4);
+}
+
+// log(t), t > 0: t = 2^e m, m in [1, 2); c_j the nearest multiple of 1/32,
+// r = m / c_j - 1 (one fma), |r| <= 1/64; log(1 + r) by its series to r^8
+// (truncation < 1e-17); absolute error about 1 ulp of the result's larger
+// part.  inf and NaN pass through.
+__device__ __forceinline__ double fast_log(double t, const lds_double* tab) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, t);
+  const uint32_t hi = (uint32_t)(b >> 32);
+  const uint32_t j = (((hi >> 14) & 63u) + 1u) >> 1;
+  const int e = (int)(hi >> 20) - 1023 + (int)(j >> 5);
+  const double m = __builtin_bit_cast(
+      double, ((uint64_t)((hi & 0x000FFFFFu) | 0x3FF00000u) << 32) | (uint32_t)b);
+  const double r = __builtin_fma(m, tab[kTabInvc + j], -1.0);
+  double q = __builtin_fma(r, -1.0 / 8, 1.0 / 7);
+  q = __builtin_fma(r, q, -1.0 / 6);
+  q = __builtin_fma(r, q, 1.0 / 5);
+  q = __builtin_fma(r, q, -1.0 / 4);
+  q = __builtin_fma(r, q, 1.0 / 3);
+  q = __builtin_fma(r, q, -0.5);
+  const double lp = __builtin_fma(r * r, q, r);
+  const double ed = (double)e;
+  const double res = __builtin_fma(ed, kLn2Hi, tab[kTabLogc + j]) + __builtin_fma(ed, kLn2Lo, lp);
+  return t < __builtin_inf() ? res : t;
+}
+
+// 1 / t: v_rcp_f64 and two Newton-Raphson steps (about 1 ulp).
+__device__ __forceinline__ double fast_rcp(double t) {
+  double r = __builtin_amdgcn_rcp(t);
+  double e = __builtin_fma(-t, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-t, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+// Fills the tables above (one workgroup's copy; threads 0..32).
+__device__ __forceinline__ void fill_f64_tables(double* tab, int tid) {
+  if (tid < 16) tab[kTabExp + tid] = exp2((double)tid * 0.0625);
+  if (tid < 33) {
+    const double invc = 1.0 / (1.0 + (double)tid * 0.03125);
+    tab[kTabInvc + tid] = invc;
+    tab[kTabLogc + tid] = tid == 32 ? 0.0 : -log(invc);
+  }
+}
 
 // One wave's part of a sweep: cells [cb, ce), subjects from s, CSR cursor ip
 // (first entry at or past cb).  Writes the wave's per-subject sums to psub
@@ -376,7 +448,8 @@ __device__ __forceinline__ void sweep_wave(const CT* __restrict__ rows,
                                         const uint32_t* __restrict__ fid,
                                         const double* __restrict__ bsrc,
                                         const double* __restrict__ lwsrc,
-                                        typename Lds<CT>::ptr ys, double* __restrict__ psub,
+                                        typename Lds<CT>::ptr ys, const lds_double* tab,
+                                        double* __restrict__ psub,
                                         double* __restrict__ pgene, uint64_t ip, uint64_t g1,
                                         int s, int k, uint32_t cb, uint32_t ce, int icpt,
                                         double gamma, int lane, int wave) {
@@ -464,7 +537,7 @@ __device__ __forceinline__ void sweep_wave(const CT* __restrict__ rows,
           CT eta = xr[u][Q];  // log offset
 #pragma unroll
           for (int q = 0; q < Q; ++q) eta += xr[u][q] * bq[q];
-          const CT ex = fast_exp(eta + c0);
+          const CT ex = fast_exp(eta + c0, tab);
           eta += bint;
           const CT t = ex + g;
           const CT inv = fast_rcp(t);
@@ -481,7 +554,7 @@ __device__ __forceinline__ void sweep_wave(const CT* __restrict__ rows,
           } else {
             const CT u_ = ex * inv;
             const CT gs = g + y;
-            const CT lt = fast_log(t);
+            const CT lt = fast_log(t, tab);
             const double pw = (double)(gs * u_ * inv);
             const double dres = (double)y - (double)gs * (double)u_;
             const double term = (double)(y * eta) - (double)gs * (double)lt;
@@ -598,6 +671,7 @@ struct Fitter {
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) sh.ysc[wave][lane + kWave * u] = 0.0f;
+    fill_f64_tables(sh.f64tab, tid);
     if (lane == 0) {
       sh.ip0[wave] = ip;
       sh.s0[wave] = (uint32_t)a;
@@ -614,7 +688,8 @@ struct Fitter {
     const CT* rows = sizeof(CT) == 4 ? (const CT*)dd.row32 : (const CT*)dd.row64;
     const CT* gy = sizeof(CT) == 4 ? (const CT*)dd.gy32 : (const CT*)dd.gy64;
     sweep_wave<CT, MODE, NB>(rows, dd.gcell, gy, dd.fid, bsrc, lwsrc,
-                             (typename Lds<CT>::ptr)(&sh.ysc[wave][0]), wk.psub, wk.pgene,
+                             (typename Lds<CT>::ptr)(&sh.ysc[wave][0]),
+                             (const lds_double*)&sh.f64tab[0], wk.psub, wk.pgene,
                              sh.ip0[wave], g1, (int)sh.s0[wave], k, block_begin(wave),
                              block_end(wave), icpt, gamma, lane, wave);
     __syncthreads();
