@@ -203,4 +203,26 @@ int nh_bobyqa_device_trace(int problem, int n, const double* x0, const double* l
   }
 }
 
+int nh_selftest_f64_math(const double* x, size_t n, int32_t which, int32_t device,
+                         double* out, char* err, size_t err_len) {
+  if ((n && (!x || !out)) || which < 0 || which > 2) {
+    put_err(err, err_len, "invalid arguments");
+    return NH_ERR_INVALID_ARGUMENT;
+  }
+  if (n == 0) return NH_OK;
+  try {
+    NH_HIP(hipSetDevice(device));
+    nh::DevBuf<double> dx, dout;
+    hipStream_t st = nullptr;
+    dx.upload(x, n, st);
+    dout.alloc(n);
+    NH_HIP(nh::launch_f64_math(dx.p, n, which, dout.p, st));
+    NH_HIP(hipMemcpy(out, dout.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return NH_OK;
+  } catch (const std::exception& e) {
+    put_err(err, err_len, e.what());
+    return NH_ERR_GPU;
+  }
+}
+
 }  // extern "C"

@@ -88,6 +88,25 @@ static hipError_t launch_batch_nb(const BatchLaunch& L, hipStream_t st) {
   }
 }
 
+__global__ __launch_bounds__(256) void f64_math_kernel(const double* x, size_t n, int which,
+                                                        double* out) {
+  __shared__ double tab[82];
+  fill_f64_tables(tab, (int)threadIdx.x);
+  __syncthreads();
+  const lds_double* t = (const lds_double*)tab;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    const double v = x[i];
+    out[i] = which == 0 ? fast_exp(v, t) : (which == 1 ? fast_log(v, t) : fast_rcp(v));
+  }
+}
+
+hipError_t launch_f64_math(const double* x, size_t n, int which, double* out, hipStream_t st) {
+  size_t blocks = (n + 255) / 256;
+  blocks = blocks < 1 ? 1 : (blocks > 1024 ? 1024 : blocks);
+  hipLaunchKernelGGL(f64_math_kernel, dim3((unsigned)blocks), dim3(256), 0, st, x, n, which, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_batch(const BatchLaunch& L, hipStream_t st) {
   if (L.n_req == 0) return hipSuccess;
   switch (L.dd.nb) {

@@ -427,6 +427,23 @@ def bobyqa_device_trace(problem, x0, lower, upper, max_trace=2000):
     return tr[: rows * (n + 1)].reshape(rows, n + 1), status
 
 
+def selftest_f64_math(x, which, device=0):
+    """The FP64 sweep's exp (which 0), log (1) or reciprocal (2) of x,
+    evaluated on the device."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    out = np.zeros_like(x)
+    err = C.create_string_buffer(512)
+    L = lib()
+    L.nh_selftest_f64_math.restype = C.c_int
+    L.nh_selftest_f64_math.argtypes = [_dp, C.c_size_t, C.c_int32, C.c_int32, _dp, C.c_char_p,
+                                       C.c_size_t]
+    rc = L.nh_selftest_f64_math(x.ctypes.data_as(_dp), len(x), which, device,
+                                out.ctypes.data_as(_dp), err, 512)
+    if rc != 0:
+        _raise(rc, err)
+    return out
+
+
 # ---------------------------------------------------------------------------
 class Session:
     """nebula_sparse_hip split at the device seam: set-up, upload and stage one

@@ -169,3 +169,11 @@ def test_everything_filtered():
     cs, subject, design, offset = _golden_inputs()
     with pytest.raises(nh.NoGenesAfterFiltering):
         nh.nebula_sparse_hip(cs, subject, design, 3, offset, nh.NebulaParams(cpc=1e9))
+
+
+def test_f64_math_selftest_argument_checks():
+    """nh_selftest_f64_math: an empty input is a no-op and an unknown function
+    is rejected, both before any device call."""
+    assert nh.selftest_f64_math(np.zeros(0), 0).size == 0
+    with pytest.raises(nh.InvalidArgument):
+        nh.selftest_f64_math(np.ones(3), 7)
