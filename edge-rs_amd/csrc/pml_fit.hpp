@@ -366,6 +366,11 @@ __device__ __forceinline__ float fast_log(float x, const lds_double*) {
 }
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// Relative size, against |loglik|, of a predicted gain and a measured loss
+// that an FP32 Newton step may show before it counts as an overshoot (the
+// FP32 sums carry rounding of order 1e-7 of the summed magnitudes).
+constexpr double kF32Trust = 1e-6;
+
 // FP64 exp, log and reciprocal of the FP64 sweeps, with short dependency
 // chains and no special-case branches (the sweep's arguments are finite;
 // overflow gives inf, underflow 0, NaN stays NaN).  Tables in LDS:
@@ -947,11 +952,13 @@ struct Fitter {
       ++step;
       newton_step(cur);
       const int tr = 1 - cur;
-      if (noise_rel > 0.0 && kn.decrement_stop) {
+      const bool fast = noise_rel > 0.0 && kn.decrement_stop;
+      double dec = NAN;
+      if (fast) {
         // f32: a step predicted to gain less than the resolution floor is
         // taken without the sweep that would measure it; the f64 finish
         // sweeps the stepped point next.
-        const double dec = decrement(cur);
+        dec = decrement(cur);
         if (dec >= 0.0 && dec <= fmax(kn.eps, noise_rel * fabs(ll))) {
           accept();
           ll_prev = ll;
@@ -964,6 +971,17 @@ struct Fitter {
       ll = sh.ll[tr];
       likdif = ll - ll_prev;
       const double noise = noise_rel * fabs(ll_prev);
+      if (fast && likdif < -noise && isfinite(ll) && dec >= 0.0 &&
+          dec <= kF32Trust * fabs(ll_prev) && likdif >= -kF32Trust * fabs(ll_prev)) {
+        // A step predicted to gain at most kF32Trust |ll| that measures as a
+        // loss no larger than that: the difference is below what FP32 sums
+        // resolve on this gene, not an overshoot.  Halving the step would
+        // only repeat the comparison; keep the step and let the f64 finish
+        // decide from there.
+        accept();
+        cur = tr;
+        break;
+      }
       backtracks = 0;
       double min_step = 40.0;
       while (likdif < -noise || isinf(ll) || (noise_rel > 0.0 && isnan(ll))) {
