@@ -52,8 +52,8 @@ struct Prepared {
 // Learned-start settings of one stage-two run.
 struct LearnedStart {
   const float* params = nullptr;  // device copy of the MLP parameter blob
-  double step = 0.05;             // BOBYQA initial step, relative to the predicted point
-  double xtol_rel = 2e-3;         // rhoend / rhobeg of the warm search
+  double step = 0.01;             // BOBYQA initial step, relative to the predicted point
+  double xtol_rel = 1e-2;         // rhoend / rhobeg of the warm search (rhoend 1e-4 x)
 };
 
 double digamma(double x);

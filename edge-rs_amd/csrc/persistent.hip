@@ -153,7 +153,7 @@ void stage_two_kernel(PersistentArgs A) {
       const double alpha = 1.0 / (exps - 1.0);
       const double lambda = 1.0 / (sqrt(exps) * (exps - 1.0));
       Fitter<NB, WAVES> f(dd, sh, wk, A.counters, S.gene, cell, alpha, lambda);
-      const FitResult r = f.run(0, (unsigned)s_order, A.knobs);
+      const FitResult r = f.run(0, (unsigned)s_order, A.knobs, s_evals > 0);
       // finite point and the lgamma tail over the count histogram
       double part = 0.0;
       bool fin = true;

@@ -1134,12 +1134,14 @@ struct Fitter {
 
   // The whole fit.  mode 0: f32 Newton then f64 finish (stage two); mode 1:
   // opt_pml in f64 from the given point (stage three).  beta in sh.beta and
-  // log w in wk.lw on entry.
-  __device__ FitResult run(int mode, unsigned ord, const FitKnobs& kn) {
+  // log w in wk.lw on entry.  same_gene: this workgroup's previous fit was of
+  // the same gene, so setup()'s per-wave CSR cursors, the zeroed count rows
+  // (every sweep leaves them zero) and the FP64 tables are still in LDS.
+  __device__ FitResult run(int mode, unsigned ord, const FitKnobs& kn, bool same_gene = false) {
     FitResult r{};
     r.finish_steps = 0;
     r.exact = 0;
-    setup();
+    if (!same_gene) setup();
     bool done = false;
     if (mode == 0) {
       FitResult f32r{};
