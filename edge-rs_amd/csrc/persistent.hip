@@ -91,13 +91,15 @@ void stage_two_kernel(PersistentArgs A) {
     if (phase == PH_WARM) {
       // The initial step is a fraction of the predicted value, widened to
       // twice the distance to stage one's estimate where the two disagree
-      // (a prediction far from stage one is less certain).  The final
-      // precision rhoend = warm_xtol * warm_step * x stays the same whatever
-      // the step: xtol is scaled down when the step is widened.
+      // (a prediction far from stage one is less certain), at most to a
+      // tenth of the predicted value: wider first steps cost more BOBYQA
+      // stages than they save where the network is right and stage one is
+      // not.  The final precision rhoend = warm_xtol * warm_step * x stays
+      // the same whatever the step: xtol is scaled down when it is widened.
       double step[2];
       for (int i = 0; i < 2; ++i) {
         const double base = A.warm_step * s_warm[i];
-        step[i] = fmax(base, 2.0 * fabs(s_warm[i] - s_cold[i]));
+        step[i] = fmax(base, fmin(2.0 * fabs(s_warm[i] - s_cold[i]), 0.1 * s_warm[i]));
         step[i] = fmin(fmax(step[i], 1e-3 * lo[i]), 0.45 * (hi[i] - lo[i]));
       }
       const double xtol = A.warm_xtol * fmin(1.0, A.warm_step * s_warm[0] / step[0]);
