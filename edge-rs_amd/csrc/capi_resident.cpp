@@ -205,7 +205,7 @@ int nh_bobyqa_device_trace(int problem, int n, const double* x0, const double* l
 
 int nh_selftest_f64_math(const double* x, size_t n, int32_t which, int32_t device,
                          double* out, char* err, size_t err_len) {
-  if ((n && (!x || !out)) || which < 0 || which > 2) {
+  if ((n && (!x || !out)) || which < 0 || which > 3) {
     put_err(err, err_len, "invalid arguments");
     return NH_ERR_INVALID_ARGUMENT;
   }

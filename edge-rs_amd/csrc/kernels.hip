@@ -6,6 +6,7 @@
 
 #include "bobyqa.hpp"
 #include "kernels.hpp"
+#include "lgamma_pos.hpp"
 #include "pml_fit.hpp"
 
 namespace nh {
@@ -96,7 +97,10 @@ __global__ __launch_bounds__(256) void f64_math_kernel(const double* x, size_t n
   const lds_double* t = (const lds_double*)tab;
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
     const double v = x[i];
-    out[i] = which == 0 ? fast_exp(v, t) : (which == 1 ? fast_log(v, t) : fast_rcp(v));
+    out[i] = which == 0 ? fast_exp(v, t)
+             : which == 1 ? fast_log(v, t)
+             : which == 2 ? fast_rcp(v)
+                          : lgamma_pos(v);
   }
 }
 

@@ -64,7 +64,8 @@ inline int persistent_waves(int nb, int waves) {
 #endif
 constexpr size_t kWorkLdsBytes = NH_WORK_LDS_BYTES;
 hipError_t launch_batch(const BatchLaunch& L, hipStream_t st);
-// out[i] = the FP64 sweep's exp / log / reciprocal (which 0 / 1 / 2) of x[i].
+// out[i] = the FP64 sweep's exp / log / reciprocal (which 0 / 1 / 2) or the
+// device objective's lgamma (3) of x[i].
 hipError_t launch_f64_math(const double* x, size_t n, int which, double* out, hipStream_t st);
 hipError_t launch_bobyqa_trace(int problem, int n, const double* x0, const double* lo,
                                const double* hi, double* trace, int max_trace, int* result,

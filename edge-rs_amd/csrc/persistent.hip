@@ -30,6 +30,7 @@
 #include "pml_fit.hpp"
 #include "persistent.hpp"
 #include "mlp.hpp"
+#include "lgamma_pos.hpp"
 #include "resident.hpp"
 
 namespace nh {
@@ -43,8 +44,8 @@ __device__ static double device_objective(double subject, double cell, const Fit
   const double alpha = 1.0 / (exps - 1.0);
   const double lambda = 1.0 / (sqrt(exps) * (exps - 1.0));
   const double base = isnan(r.ll) ? r.ll_prev : r.ll;
-  double v = base + n_cells * cell * log(cell) + k * alpha * log(lambda) - k * lgamma(alpha);
-  v += tail - (n_pos - n_one - n_two) * lgamma(cell) + (n_one + n_two) * log(cell) +
+  double v = base + n_cells * cell * log(cell) + k * alpha * log(lambda) - k * lgamma_pos(alpha);
+  v += tail - (n_pos - n_one - n_two) * lgamma_pos(cell) + (n_one + n_two) * log(cell) +
        n_two * log(cell + 1.0);
   v += -0.5 * r.log_det + log(1.0 + r.second);
   return isfinite(v) ? -v : INFINITY;
@@ -76,6 +77,7 @@ void stage_two_kernel(PersistentArgs A) {
   __shared__ int s_slot, s_active, s_found, s_failed, s_have_warm, s_evals, s_order, s_ok;
   __shared__ int s_phase, s_wfailed, s_rejected, s_from_warm;
   __shared__ double s_subject, s_cell, s_value, s_wx[2], s_wf, s_cold[2], s_warm[2];
+  __shared__ double s_anchor_f;  // objective at the inner fits' current start
   extern __shared__ double nh_s2_dyn[];
   const int tid = threadIdx.x;
   const DeviceData& dd = A.dd;
@@ -162,7 +164,7 @@ void stage_two_kernel(PersistentArgs A) {
       for (int s = tid; s < k; s += NT) fin = fin && isfinite(wk.lw[s]);
       if (tid < NB) fin = fin && isfinite(sh.beta[tid]);
       for (uint32_t i = (uint32_t)tid; i < S.tail_len; i += NT)
-        part += A.tail_mult[S.tail_off + i] * lgamma(A.tail_val[S.tail_off + i] + cell);
+        part += A.tail_mult[S.tail_off + i] * lgamma_pos(A.tail_val[S.tail_off + i] + cell);
       part = wave_sum(part);
       const bool wfin = __all(fin);
       if ((tid % 64) == 0) red[tid / 64] = wfin ? part : NAN;
@@ -180,7 +182,11 @@ void stage_two_kernel(PersistentArgs A) {
                                S.n_one, S.n_two);
         s_value = v;
         s_evals += 1;
-        s_ok = isfinite(v) && !s_have_warm;
+        // Inner fits start from the gene's first finite fit (the cold search,
+        // as edge-rs does); the warm search moves that start to its best
+        // point so far, which its steps stay close to.
+        s_ok = isfinite(v) && (!s_have_warm || (s_phase == PH_WARM && v < s_anchor_f));
+        if (s_ok) s_anchor_f = v;
       }
       __syncthreads();
       if (s_ok) {

@@ -285,9 +285,10 @@ int nh_bobyqa_device_trace(int problem, int n, const double* x0,
                            double* trace, int max_trace, int* evals_out,
                            char* err, size_t err_len);
 
-/* The FP64 sweep's exp (which 0), log (1) and reciprocal (2) applied on the
- * device to x[0..n), for numerics tests of the device fit (no counterpart in
- * edge-rs, whose fits call the Rust standard library's f64 functions).
+/* The FP64 sweep's exp (which 0), log (1) and reciprocal (2), and the device
+ * objective's log Gamma for x > 0 (3), applied on the device to x[0..n), for
+ * numerics tests of the device fit (no counterpart in edge-rs, whose fits call
+ * the Rust standard library's and statrs's f64 functions).
  * Returns NH_OK or an NH_ERR_* code. */
 int nh_selftest_f64_math(const double* x, size_t n, int32_t which, int32_t device,
                          double* out, char* err, size_t err_len);

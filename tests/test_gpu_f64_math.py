@@ -1,5 +1,6 @@
-"""The FP64 sweep's table-driven exp and log and its Newton-refined
-reciprocal, run on the device through the C ABI, against numpy's libm: exp
+"""The FP64 sweep's table-driven exp and log, its Newton-refined
+reciprocal and the device objective's log Gamma, run on the device through
+the C ABI, against numpy's libm and scipy: exp
 within 2 ulp, reciprocal within 1 ulp, log within 2 ulp of its larger part
 (relative accuracy kept within 1/256 of t = 1, where log 1 = 0 exactly),
 overflow, underflow, inf and NaN as libm gives them.  Marked gpu."""
@@ -65,3 +66,18 @@ def test_reciprocal_matches_libm():
     got = nh.selftest_f64_math(t, 2)
     assert _ulp_err(got, 1.0 / t).max() <= 1.0
     assert np.isnan(nh.selftest_f64_math(np.array([np.nan]), 2)[0])
+
+
+def test_lgamma_matches_scipy():
+    """The device objective's log Gamma (shift to x >= 10, Stirling's series)
+    against scipy's gammaln over the arguments the objective uses: alpha and
+    the cell parameter over their boxes, count values + cell."""
+    from scipy.special import gammaln
+    rng = np.random.default_rng(14)
+    x = np.concatenate([np.exp(rng.uniform(np.log(1e-4), np.log(1e4), 300_000)),
+                        3.0 + rng.integers(0, 500, 200_000) + rng.uniform(1e-4, 1e3, 200_000),
+                        [1e-4, 0.5, 1.0, 2.0, 3.0, 9.999999, 10.0, 10.000001, 1e4]])
+    got = nh.selftest_f64_math(x, 3)
+    ref = gammaln(x)
+    err = np.abs(got - ref) / np.maximum(1.0, np.abs(ref))
+    assert err.max() < 2e-14, (err.max(), x[np.argmax(err)])
