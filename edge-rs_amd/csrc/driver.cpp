@@ -1087,8 +1087,7 @@ int nh_session_autotune(nh_session_t* s, int32_t engine, int32_t* wg_size, doubl
   int seen[3] = {0, 0, 0};
   for (int c = 0; c < 3; ++c) {
     // sizes that launch the same kernel as an earlier one are not rerun
-    const int w = engine == NH_ENGINE_PLAIN ? effective_waves((int)s->P.n_coef, sizes[c] / 64)
-                                            : persistent_waves((int)s->P.n_coef, sizes[c] / 64);
+    const int w = effective_waves((int)s->P.n_coef, sizes[c] / 64);
     bool dup = false;
     for (int d = 0; d < c; ++d) dup = dup || seen[d] == w;
     seen[c] = w;

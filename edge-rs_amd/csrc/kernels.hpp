@@ -48,15 +48,6 @@ inline int effective_waves(int nb, int waves) {
   return waves <= 4 ? 4 : 8;
 }
 
-// The same for the persistent stage-two kernel, which is built for 4 and 16
-// waves per gene when p <= 4 (an 8-wave request runs with 16): its 8-wave,
-// 128-VGPR instance returned objective values that disagree with the f64
-// oracle, while the same source at 4 or 16 waves, or at 8 waves with a
-// 256-VGPR budget, agrees to 1e-10.
-inline int persistent_waves(int nb, int waves) {
-  const int w = effective_waves(nb, waves);
-  return (nb <= 4 && w == 8) ? 16 : w;
-}
 // Largest per-workgroup work area placed in LDS (bytes); larger ones live in
 // global memory.
 #ifndef NH_WORK_LDS_BYTES

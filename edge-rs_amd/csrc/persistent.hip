@@ -285,11 +285,11 @@ static hipError_t launch_t(const PersistentArgs& A, int grid, hipStream_t st) {
 
 template <int NB>
 static hipError_t launch_nb(const PersistentArgs& A, int waves, int grid, hipStream_t st) {
+  if (waves == 4) return launch_t<NB, 4>(A, grid, st);
   if constexpr (NB <= 4) {
-    return waves == 4 ? launch_t<NB, 4>(A, grid, st) : launch_t<NB, 16>(A, grid, st);
-  } else {
-    return waves == 4 ? launch_t<NB, 4>(A, grid, st) : launch_t<NB, 8>(A, grid, st);
+    if (waves == 16) return launch_t<NB, 16>(A, grid, st);
   }
+  return launch_t<NB, 8>(A, grid, st);
 }
 
 hipError_t launch_persistent(const PersistentArgs& A, int waves, int grid, hipStream_t st) {
@@ -360,7 +360,7 @@ void run_stage_two_batched(Prepared& P, Resident& R, const nebula_hip_options_t&
     waves = 4;
   else if (o.wg_size == 0 && (int)n <= 4 * dev_cus)
     waves = 16;
-  waves = persistent_waves((int)nb, waves);
+  waves = effective_waves((int)nb, waves);
   const int per_cu = waves == 16 ? 1 : (waves == 8 ? 2 : 4);
   const int grid = (int)std::min<size_t>(n, (size_t)std::max(1, dev_cus * per_cu));
   DevBuf<DevSearch> dsearch;
