@@ -352,13 +352,14 @@ void run_stage_two_batched(Prepared& P, Resident& R, const nebula_hip_options_t&
   NH_HIP(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, R.device));
   // tile shape: one gene per workgroup; wide workgroups when genes are few
   // waves per gene: the tuned or requested size; otherwise 16 (a whole CU per
-  // gene) while genes are few per CU, when the slowest searches set the end
+  // gene) when there are no more genes than CUs, else 8 (two genes per CU
+  // overlap one's serial steps with the other's sweeps)
   int waves = 8;
   if (o.wg_size >= 1024)
     waves = 16;
   else if (o.wg_size > 0 && o.wg_size <= 256)
     waves = 4;
-  else if (o.wg_size == 0 && (int)n <= 4 * dev_cus)
+  else if (o.wg_size == 0 && (int)n <= dev_cus)
     waves = 16;
   waves = effective_waves((int)nb, waves);
   const int per_cu = waves == 16 ? 1 : (waves == 8 ? 2 : 4);
