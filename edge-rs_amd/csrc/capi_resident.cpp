@@ -7,6 +7,7 @@
 
 #include "../../include/nebula_hip.h"
 #include "bobyqa.hpp"
+#include "driver.hpp"
 #include "resident.hpp"
 
 struct nh_resident {
@@ -98,19 +99,25 @@ int nh_resident_upload(size_t n_cells, size_t n_coef, size_t n_subjects, const d
   return NH_OK;
 }
 
-int nh_resident_solve(nh_resident_t* h, const nh_request_t* requests, size_t n_req, double eps,
-                      int32_t mode, nh_reply_t* replies, char* err, size_t err_len) {
-  if (!h) {
-    put_err(err, err_len, "null handle");
+}  // extern "C"
+
+namespace {
+
+// Checks the requests and converts them to the device layout; returns NH_OK
+// or an error code with the message in err.
+int build_requests(const nh::Resident& r, const nh_request_t* requests, size_t n_req,
+                   std::vector<nh::DevRequest>& req, std::vector<double>& bi,
+                   std::vector<double>& lwi, bool& any_lw, char* err, size_t err_len) {
+  const size_t nb = r.nb, k = r.k;
+  if (n_req && !requests) {
+    put_err(err, err_len, "null requests");
     return NH_ERR_INVALID_ARGUMENT;
   }
-  nh::Resident& r = h->r;
-  const size_t nb = r.nb, k = r.k;
-  std::vector<nh::DevRequest> req(n_req);
-  std::vector<double> bi(n_req * nb), lwi;
-  bool any_lw = false;
+  req.assign(n_req, nh::DevRequest{});
+  bi.assign(n_req * nb, 0.0);
+  any_lw = false;
   for (size_t q = 0; q < n_req; ++q) any_lw = any_lw || requests[q].log_w_init;
-  if (any_lw) lwi.assign(n_req * k, 0.0);
+  lwi.assign(any_lw ? n_req * k : 0, 0.0);
   for (size_t q = 0; q < n_req; ++q) {
     const nh_request_t& x = requests[q];
     if (x.gene >= r.n_genes || x.ord < 1 || x.ord > 3 || !x.beta_init) {
@@ -131,10 +138,62 @@ int nh_resident_solve(nh_resident_t* h, const nh_request_t* requests, size_t n_r
     if (any_lw && x.log_w_init)
       for (size_t s = 0; s < k; ++s) lwi[q * k + s] = x.log_w_init[s];
   }
+  return NH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nh_resident_submit(nh_resident_t* h, const nh_request_t* requests, size_t n_req, double eps,
+                       int32_t mode, char* err, size_t err_len) {
+  if (!h) {
+    put_err(err, err_len, "null handle");
+    return NH_ERR_INVALID_ARGUMENT;
+  }
+  if (mode != NH_SOLVE_F32_FINISH && mode != NH_SOLVE_F64) {
+    put_err(err, err_len, "unknown solve mode");
+    return NH_ERR_INVALID_ARGUMENT;
+  }
+  if (h->r.pending_launched) {
+    put_err(err, err_len, "a launch is already in flight on this handle; collect it first");
+    return NH_ERR_INVALID_ARGUMENT;
+  }
+  std::vector<nh::DevRequest> req;
+  std::vector<double> bi, lwi;
+  bool any_lw = false;
+  const int rc = build_requests(h->r, requests, n_req, req, bi, lwi, any_lw, err, err_len);
+  if (rc != NH_OK) return rc;
+  try {
+    h->r.submit(req, bi, any_lw ? &lwi : nullptr, eps, mode, 8);
+  } catch (const std::exception& e) {
+    put_err(err, err_len, e.what());
+    return NH_ERR_GPU;
+  }
+  return NH_OK;
+}
+
+int nh_resident_collect(nh_resident_t* h, nh_reply_t* replies, size_t n_req, char* err,
+                        size_t err_len) {
+  if (!h) {
+    put_err(err, err_len, "null handle");
+    return NH_ERR_INVALID_ARGUMENT;
+  }
+  nh::Resident& r = h->r;
+  if (!r.pending_launched) {
+    put_err(err, err_len, "no launch in flight on this handle");
+    return NH_ERR_INVALID_ARGUMENT;
+  }
+  if (n_req != r.in_flight() || (n_req && !replies)) {
+    put_err(err, err_len, "replies must match the submitted request count (" +
+                              std::to_string(r.in_flight()) + ")");
+    return NH_ERR_INVALID_ARGUMENT;
+  }
+  const size_t nb = r.nb, k = r.k;
   std::vector<nh::DevReply> out;
   std::vector<double> ob, olw, oinfo;
   try {
-    r.solve(req, bi, any_lw ? &lwi : nullptr, eps, mode, 8, out, ob, olw, &oinfo);
+    r.collect(out, ob, olw, &oinfo);
   } catch (const std::exception& e) {
     put_err(err, err_len, e.what());
     return NH_ERR_GPU;
@@ -148,12 +207,19 @@ int nh_resident_solve(nh_resident_t* h, const nh_request_t* requests, size_t n_r
     o.iterations = out[q].iterations;
     o.backtracks = out[q].backtracks;
     o.finish_steps = out[q].finish_steps;
-    o.convergence = 0;
+    o.convergence = nh::fit_convergence(out[q], &oinfo[q * nb * nb], nb, false, NH_CONV_SUCCESS);
     if (o.beta) std::memcpy(o.beta, &ob[q * nb], nb * sizeof(double));
     if (o.log_w) std::memcpy(o.log_w, &olw[q * k], k * sizeof(double));
     if (o.information) std::memcpy(o.information, &oinfo[q * nb * nb], nb * nb * sizeof(double));
   }
   return NH_OK;
+}
+
+int nh_resident_solve(nh_resident_t* h, const nh_request_t* requests, size_t n_req, double eps,
+                      int32_t mode, nh_reply_t* replies, char* err, size_t err_len) {
+  const int rc = nh_resident_submit(h, requests, n_req, eps, mode, err, err_len);
+  if (rc != NH_OK) return rc;
+  return nh_resident_collect(h, replies, n_req, err, err_len);
 }
 
 void nh_resident_free(nh_resident_t* h) { delete h; }

@@ -108,6 +108,25 @@ bool cholesky_inverse(const std::vector<double>& a, size_t n, std::vector<double
 
 }  // namespace
 
+int fit_convergence(const DevReply& r, const double* information, size_t nb, bool at_bound,
+                    int initial) {
+  int code = initial;
+  if (code == NH_CONV_SUCCESS) {
+    if (at_bound)
+      code = NH_CONV_AT_BOUND;
+    else if (std::isnan(r.ll))
+      code = NH_CONV_NOT_FINITE;
+    else if (r.iterations == kFitMaxIter)
+      code = NH_CONV_MAX_ITER;
+    else if (r.backtracks == kFitMaxBacktrack + 1)
+      code = NH_CONV_CRITICAL_POINT;
+    else if (r.backtracks == kFitMaxBacktrack + 2)
+      code = NH_CONV_STALLED;
+  }
+  if (nb > 1 && min_eigenvalue(information, nb) < 1e-8) code = NH_CONV_SINGULAR;
+  return code;
+}
+
 // ---------------------------------------------------------------------------
 // ptmg value / gradient / Hessian from the device sums (ptmg.rs:288-478,
 // 742-896, 1061-1252).
@@ -837,28 +856,9 @@ void run_stage_three(Prepared& P, Resident& R, int waves, nebula_fit_t* out) {
   scale[P.intercept] = 1.0;
   for (size_t i = 0; i < G; ++i) {
     const DevReply& r = rep[i];
-    // check_convergence (pml.rs:1111-1139) with opt_pml's own grading first
-    int code = NH_CONV_SUCCESS;
-    if (std::isnan(r.ll))
-      code = NH_CONV_NOT_FINITE;
-    else if (r.iterations == 50)
-      code = NH_CONV_MAX_ITER;
-    else if (r.backtracks == 11)
-      code = NH_CONV_CRITICAL_POINT;
-    else if (r.backtracks == 12)
-      code = NH_CONV_STALLED;
     const double* info = &oinfo[i * nb * nb];
     const bool at_bound = sig[i] == p.max_subject || gam[i] == p.min_cell;
-    int c2 = initial[i];
-    if (c2 == NH_CONV_SUCCESS) {
-      c2 = at_bound ? NH_CONV_AT_BOUND
-                    : (std::isnan(r.ll) ? NH_CONV_NOT_FINITE
-                                        : (r.iterations == 50 ? NH_CONV_MAX_ITER
-                                                              : (r.backtracks == 11 ? NH_CONV_CRITICAL_POINT
-                                                                                    : (r.backtracks == 12 ? NH_CONV_STALLED : NH_CONV_SUCCESS))));
-    }
-    (void)code;
-    if (nb > 1 && min_eigenvalue(info, nb) < 1e-8) c2 = NH_CONV_SINGULAR;
+    int c2 = fit_convergence(r, info, nb, at_bound, initial[i]);
     std::vector<double> infov(info, info + nb * nb), inv;
     std::vector<double> gcov(packed), gse(nb);
     if (c2 != NH_CONV_SINGULAR && cholesky_inverse(infov, nb, inv)) {

@@ -49,6 +49,15 @@ struct Prepared {
   std::vector<float> features;  // kept x 12 warm-start features (after stage one)
 };
 
+// check_convergence (src/sc/pml.rs:1111-1139) on a device fit's reply:
+// initial is the outer search's code (NH_CONV_SUCCESS when it succeeded),
+// at_bound whether a variance component sits on its bound; information is
+// the nb x nb information matrix of the fit.
+struct DevReply;
+constexpr int kFitMaxIter = 50, kFitMaxBacktrack = 10;
+int fit_convergence(const DevReply& r, const double* information, size_t nb, bool at_bound,
+                    int initial);
+
 // Learned-start settings of one stage-two run.
 struct LearnedStart {
   const float* params = nullptr;  // device copy of the MLP parameter blob

@@ -1,4 +1,7 @@
 // Host side of the resident batch: upload once, solve many request batches.
+#include <stdexcept>
+
+#include "driver.hpp"
 #include "resident.hpp"
 
 #include <algorithm>
@@ -120,18 +123,29 @@ void Resident::solve(const std::vector<DevRequest>& r, const std::vector<double>
                      std::vector<DevReply>& out, std::vector<double>& ob,
                      std::vector<double>& olw, std::vector<double>* oinfo,
                      bool decrement_stop) {
+  submit(r, bi, lwi, eps, mode, waves, decrement_stop);
+  collect(out, ob, olw, oinfo);
+}
+
+void Resident::submit(const std::vector<DevRequest>& r, const std::vector<double>& bi,
+                      const std::vector<double>* lwi, double eps, int mode, int waves,
+                      bool decrement_stop) {
+  if (pending_launched) throw std::logic_error("a launch is already in flight on this handle");
   const size_t n = r.size();
-  out.resize(n);
-  ob.resize(n * nb);
-  olw.resize(n * k);
-  if (oinfo) oinfo->resize(n * nb * nb);
-  if (n == 0) return;
   for (const DevRequest& q : r)
     if (q.gene >= n_genes) throw std::invalid_argument("request names a gene that is not resident");
+  if (n == 0) {
+    pending = 0;
+    pending_launched = true;
+    return;
+  }
+  h_req = r;
+  h_beta = bi;
+  if (lwi) h_lw = *lwi;
   NH_HIP(hipSetDevice(device));
-  req.upload(r.data(), n, stream);
-  beta_init.upload(bi.data(), n * nb, stream);
-  if (lwi) lw_init.upload(lwi->data(), n * k, stream);
+  req.upload(h_req.data(), n, stream);
+  beta_init.upload(h_beta.data(), n * nb, stream);
+  if (lwi) lw_init.upload(h_lw.data(), n * k, stream);
   reply.alloc(n);
   out_beta.alloc(n * nb);
   out_lw.alloc(n * k);
@@ -153,8 +167,8 @@ void Resident::solve(const std::vector<DevRequest>& r, const std::vector<double>
   L.work_in_lds = in_lds ? 1 : 0;
   L.knobs.eps = eps;
   L.knobs.noise_rel = 1e-8;
-  L.knobs.max_iter = 50;
-  L.knobs.max_backtrack = 10;
+  L.knobs.max_iter = kFitMaxIter;
+  L.knobs.max_backtrack = kFitMaxBacktrack;
   L.knobs.decrement_stop = decrement_stop ? 1 : 0;
   L.mode = mode;
   L.waves = waves;
@@ -163,6 +177,22 @@ void Resident::solve(const std::vector<DevRequest>& r, const std::vector<double>
   NH_HIP(hipEventRecord(ev0, stream));
   NH_HIP(launch_batch(L, stream));
   NH_HIP(hipEventRecord(ev1, stream));
+  pending = n;
+  pending_launched = true;
+}
+
+void Resident::collect(std::vector<DevReply>& out, std::vector<double>& ob,
+                       std::vector<double>& olw, std::vector<double>* oinfo) {
+  if (!pending_launched) throw std::logic_error("no launch in flight on this handle");
+  const size_t n = pending;
+  pending = 0;
+  pending_launched = false;
+  out.resize(n);
+  ob.resize(n * nb);
+  olw.resize(n * k);
+  if (oinfo) oinfo->resize(n * nb * nb);
+  if (n == 0) return;
+  NH_HIP(hipSetDevice(device));
   NH_HIP(hipMemcpyAsync(out.data(), reply.p, n * sizeof(DevReply), hipMemcpyDeviceToHost, stream));
   NH_HIP(hipMemcpyAsync(ob.data(), out_beta.p, n * nb * sizeof(double), hipMemcpyDeviceToHost,
                         stream));

@@ -65,6 +65,11 @@ struct Resident {
   DevBuf<DevReply> reply;
   DevBuf<double> beta_init, lw_init, out_beta, out_lw, out_info, scratch;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // the launch in flight: request count and host copies of its inputs
+  size_t pending = 0;
+  bool pending_launched = false;
+  std::vector<DevRequest> h_req;
+  std::vector<double> h_beta, h_lw;
   double kernel_ms = 0.0;
   unsigned long long launches = 0;
 
@@ -73,13 +78,24 @@ struct Resident {
               const uint64_t* subject_start, size_t n_genes, const uint64_t* gene_ptr,
               const uint32_t* cells, const double* counts, int device);
   DeviceData view() const;
-  // Runs n requests, replies filled on the host.  decrement_stop: mode 0
-  // fits stop on the Newton decrement (stage-two evaluations).
+  // Runs n requests, replies filled on the host (submit + collect).
+  // decrement_stop: mode 0 fits stop on the Newton decrement (stage-two
+  // evaluations).
   void solve(const std::vector<DevRequest>& r, const std::vector<double>& beta_init,
              const std::vector<double>* lw_init, double eps, int mode, int waves,
              std::vector<DevReply>& out, std::vector<double>& out_beta,
              std::vector<double>& out_lw, std::vector<double>* out_info,
              bool decrement_stop = false);
+  // Uploads n requests and launches their fits on the handle's stream without
+  // waiting; one launch may be in flight (std::logic_error otherwise).  The
+  // request arrays are copied, so the caller's may go at once.
+  void submit(const std::vector<DevRequest>& r, const std::vector<double>& beta_init,
+              const std::vector<double>* lw_init, double eps, int mode, int waves,
+              bool decrement_stop = false);
+  // Waits for the launch in flight and copies its replies out.
+  void collect(std::vector<DevReply>& out, std::vector<double>& out_beta,
+               std::vector<double>& out_lw, std::vector<double>* out_info);
+  size_t in_flight() const { return pending; }
   FitCounters read_counters();
   void reset_counters();
 };

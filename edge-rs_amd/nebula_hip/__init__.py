@@ -153,6 +153,12 @@ def lib():
         L.nh_resident_solve.restype = C.c_int
         L.nh_resident_solve.argtypes = [C.c_void_p, C.POINTER(_Request), C.c_size_t, C.c_double,
                                         C.c_int32, C.POINTER(_Reply), C.c_char_p, C.c_size_t]
+        L.nh_resident_submit.restype = C.c_int
+        L.nh_resident_submit.argtypes = [C.c_void_p, C.POINTER(_Request), C.c_size_t,
+                                         C.c_double, C.c_int32, C.c_char_p, C.c_size_t]
+        L.nh_resident_collect.restype = C.c_int
+        L.nh_resident_collect.argtypes = [C.c_void_p, C.POINTER(_Reply), C.c_size_t, C.c_char_p,
+                                          C.c_size_t]
         L.nh_resident_free.argtypes = [C.c_void_p]
         L.nh_bobyqa.restype = C.c_int
         L.nh_bobyqa.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_int, OBJECTIVE_FN,
@@ -337,7 +343,15 @@ class ResidentBatch:
 
     def solve(self, genes, subject, cell, beta_init, log_w_init=None, ord=1, eps=1e-6,
               mode=SOLVE_F32_FINISH):
-        """One penalised fit per request; returns a dict of arrays."""
+        """One penalised fit per request; returns a dict of arrays
+        (nh_resident_solve: submit, then collect)."""
+        self.submit(genes, subject, cell, beta_init, log_w_init, ord, eps, mode)
+        return self.collect()
+
+    def submit(self, genes, subject, cell, beta_init, log_w_init=None, ord=1, eps=1e-6,
+               mode=SOLVE_F32_FINISH):
+        """Launches the fits and returns at once (nh_resident_submit); one
+        launch may be in flight, and collect() returns its replies."""
         n = len(genes)
         nb, k = self.n_coef, self.k
         bi = np.ascontiguousarray(np.broadcast_to(np.asarray(beta_init, dtype=np.float64),
@@ -350,10 +364,6 @@ class ResidentBatch:
         subject = np.broadcast_to(np.asarray(subject, dtype=np.float64), (n,))
         cell = np.broadcast_to(np.asarray(cell, dtype=np.float64), (n,))
         reqs = (_Request * n)()
-        beta = np.zeros((n, nb))
-        logw = np.zeros((n, k))
-        info = np.zeros((n, nb * nb))
-        reps = (_Reply * n)()
         for q in range(n):
             reqs[q].gene = int(genes[q])
             reqs[q].ord = int(ords[q])
@@ -361,13 +371,30 @@ class ResidentBatch:
             reqs[q].cell = float(cell[q])
             reqs[q].beta_init = bi[q].ctypes.data_as(_dp)
             reqs[q].log_w_init = lw[q].ctypes.data_as(_dp) if lw is not None else None
+        err = C.create_string_buffer(512)
+        rc = lib().nh_resident_submit(self._h, reqs, n, eps, mode, err, 512)
+        if rc != 0:
+            _raise(rc, err)
+        self._pending = n
+
+    def collect(self, n=None):
+        """Waits for the launch in flight and returns its replies as a dict of
+        arrays (nh_resident_collect); n defaults to the submitted count."""
+        n = getattr(self, "_pending", 0) if n is None else n
+        nb, k = self.n_coef, self.k
+        beta = np.zeros((n, nb))
+        logw = np.zeros((n, k))
+        info = np.zeros((n, nb * nb))
+        reps = (_Reply * n)()
+        for q in range(n):
             reps[q].beta = beta[q].ctypes.data_as(_dp)
             reps[q].log_w = logw[q].ctypes.data_as(_dp)
             reps[q].information = info[q].ctypes.data_as(_dp)
         err = C.create_string_buffer(512)
-        rc = lib().nh_resident_solve(self._h, reqs, n, eps, mode, reps, err, 512)
+        rc = lib().nh_resident_collect(self._h, reps, n, err, 512)
         if rc != 0:
             _raise(rc, err)
+        self._pending = 0
         return dict(
             beta=beta, log_w=logw, information=info,
             log_likelihood=np.array([r.log_likelihood for r in reps]),
@@ -376,7 +403,8 @@ class ResidentBatch:
             second_order=np.array([r.second_order for r in reps]),
             iterations=np.array([r.iterations for r in reps]),
             backtracks=np.array([r.backtracks for r in reps]),
-            finish_steps=np.array([r.finish_steps for r in reps]))
+            finish_steps=np.array([r.finish_steps for r in reps]),
+            convergence=np.array([r.convergence for r in reps]))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -260,8 +260,25 @@ int nh_resident_upload(size_t n_cells, size_t n_coef, size_t n_subjects,
                        const double* counts, int32_t device,
                        nh_resident_t** out, char* err, size_t err_len);
 
-/* Solves n_req requests (submit + collect).  eps is nebula's absolute
- * stopping tolerance; mode is NH_SOLVE_*. */
+/* ResidentBatch::submit (src/gpu/nebula_gpu.rs:466-530): checks and uploads
+ * n_req requests and launches their fits on the handle's stream, returning
+ * without waiting (the request arrays may be freed at once).  eps is
+ * nebula's absolute stopping tolerance; mode is NH_SOLVE_*.  One launch may
+ * be in flight per handle: a second submit before its collect is
+ * NH_ERR_INVALID_ARGUMENT. */
+int nh_resident_submit(nh_resident_t* r, const nh_request_t* requests, size_t n_req,
+                       double eps, int32_t mode, char* err, size_t err_len);
+
+/* ResidentBatch::collect (src/gpu/nebula_gpu.rs:605-648): waits for the
+ * launch in flight and writes one reply per request, in request order;
+ * n_req must be the submitted count.  A reply's convergence is
+ * check_convergence (src/sc/pml.rs:1111-1139) on the fit alone: 1, -10
+ * (critical point), -20 (iteration budget), -25 (singular information),
+ * -30 (not finite) or -40 (stalled). */
+int nh_resident_collect(nh_resident_t* r, nh_reply_t* replies, size_t n_req, char* err,
+                        size_t err_len);
+
+/* nh_resident_submit then nh_resident_collect. */
 int nh_resident_solve(nh_resident_t* r, const nh_request_t* requests,
                       size_t n_req, double eps, int32_t mode,
                       nh_reply_t* replies, char* err, size_t err_len);

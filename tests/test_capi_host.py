@@ -177,3 +177,13 @@ def test_f64_math_selftest_argument_checks():
     assert nh.selftest_f64_math(np.zeros(0), 0).size == 0
     with pytest.raises(nh.InvalidArgument):
         nh.selftest_f64_math(np.ones(3), 7)
+
+
+def test_resident_submit_and_collect_reject_a_null_handle():
+    """nh_resident_submit / nh_resident_collect check the handle before any
+    device call."""
+    L = nh.lib()
+    err = C.create_string_buffer(256)
+    assert L.nh_resident_submit(None, None, 0, 1e-6, 0, err, 256) == nh.InvalidArgument.code
+    assert b"null handle" in err.value
+    assert L.nh_resident_collect(None, None, 0, err, 256) == nh.InvalidArgument.code

@@ -58,6 +58,55 @@ def test_device_f64_fit_reproduces_nebula_values():
     assert r["iterations"][1] == 9
     np.testing.assert_allclose(r["beta"][2], [4.9609375, -4.9609375], rtol=1e-11)
     assert (r["iterations"][2], r["backtracks"][2]) == (3, 11)
+    # check_convergence codes of the same fits (src/sc/pml.rs:1111-1139)
+    assert r["convergence"].tolist() == [1, 1, -10]
+
+
+def test_device_f64_fit_convergence_codes():
+    """The reply's convergence code for the oracle's -40 (stalled), -20
+    (iteration budget, eps < 0) and -25 (collinear design) fixtures
+    (src/sc/pml.rs:2151-2700), with the oracle's iterations and backtracks."""
+    rb = nh.ResidentBatch(DESIGN, OFFSET, START, [0, len(COUNTS)], CELLS, COUNTS)
+    r = rb.solve([0], [8.0], [0.001], [[20.0, 0.0]], mode=nh.ResidentBatch.SOLVE_F64)
+    o = oracle.opt_pml(DESIGN, OFFSET, COUNTS, CELLS.astype(np.uint64), START,
+                       np.array([10.0, 12.0, 8.0]), [20.0, 0.0], 8.0, 0.001)
+    assert (r["iterations"][0], r["backtracks"][0], r["convergence"][0]) == (2, 12, -40)
+    assert (o["iterations"], o["backtracks"], o["convergence"]) == (2, 12, -40)
+    r = rb.solve([0], [0.25], [2.0], [[0.5, -0.25]], eps=-1e-300, mode=nh.ResidentBatch.SOLVE_F64)
+    assert (r["iterations"][0], r["convergence"][0]) == (50, -20)
+    rb.close()
+    design = np.column_stack([DESIGN, DESIGN[:, 1]])
+    rb = nh.ResidentBatch(design, OFFSET, START, [0, len(COUNTS)], CELLS, COUNTS)
+    r = rb.solve([0], [0.25], [2.0], [[0.5, -0.25, 0.1]], mode=nh.ResidentBatch.SOLVE_F64)
+    assert r["convergence"][0] == -25
+    rb.close()
+
+
+def test_resident_submit_collect_matches_solve():
+    """submit returns with the launch in flight; collect gives the same
+    replies as solve; a second submit before collect and a collect with the
+    wrong count are rejected, and the handle stays usable."""
+    w, d, lo, fid = synthetic_gene_batch()
+    rb = nh.ResidentBatch(d, lo, fid, w.indptr.astype(np.uint64), w.indices, w.data)
+    genes = np.arange(w.n_genes)
+    subj = np.linspace(0.05, 0.6, w.n_genes)
+    cell = np.linspace(0.3, 4.0, w.n_genes)
+    beta0 = np.zeros((w.n_genes, w.n_coef))
+    beta0[:, 0] = -1.0
+    a = rb.solve(genes, subj, cell, beta0)
+    rb.submit(genes, subj, cell, beta0)
+    with pytest.raises(nh.InvalidArgument):
+        rb.submit(genes, subj, cell, beta0)
+    with pytest.raises(nh.InvalidArgument):
+        rb.collect(n=len(genes) - 1)
+    b = rb.collect()
+    for key in a:
+        np.testing.assert_array_equal(a[key], b[key])
+    with pytest.raises(nh.InvalidArgument):
+        rb.collect()
+    c = rb.solve(genes, subj, cell, beta0)
+    np.testing.assert_array_equal(a["log_likelihood"], c["log_likelihood"])
+    rb.close()
 
 
 def synthetic_gene_batch(n_cells=6000, k=12, p=4, n_genes=6, seed=7):
