@@ -62,7 +62,7 @@ int fit_convergence(const DevReply& r, const double* information, size_t nb, boo
 struct LearnedStart {
   const float* params = nullptr;  // device copy of the MLP parameter blob
   double step = 0.01;             // BOBYQA initial step, relative to the predicted point
-  double xtol_rel = 1e-2;         // rhoend / rhobeg of the warm search (rhoend 1e-4 x)
+  double xtol_rel = 3e-2;         // rhoend / rhobeg of the warm search (rhoend 3e-4 x)
 };
 
 double digamma(double x);

@@ -113,7 +113,7 @@ typedef struct {
   int32_t host_threads;  /* 0 = hardware concurrency */
   double learned_step;   /* warm search's initial BOBYQA step relative to the
                             predicted point (0 = default 0.01) */
-  double learned_xtol;   /* warm search's rhoend / rhobeg (0 = default 1e-2) */
+  double learned_xtol;   /* warm search's rhoend / rhobeg (0 = default 3e-2) */
 } nebula_hip_options_t;
 
 /* Per-phase timings and counters of one call (EDGE_RS_GPU_TIMING's phases,
