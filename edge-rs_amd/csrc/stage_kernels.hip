@@ -5,7 +5,8 @@
 //
 // Three passes over the gene's cells, each the coalesced 64-cell row walk of
 // pml_fit.hpp with the positive counts merged through LDS:
-//   1. e = exp(eta) per cell: per-subject sum of e, per-gene sum of y * eta;
+//   1. e = exp(eta) per cell: per-subject sum of e, per-gene sum of y * eta
+//      (exp, log and 1/w by the FP64 sweep's table-driven versions);
 //   2. with ym = (Y_s + alpha) / (sum e + lambda) per subject: the gradient
 //      sums (and the Hessian's per-subject vectors when asked);
 //   3. Hessian only: the Gram  sum_i W_i x_i x_i'  with
@@ -151,6 +152,7 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_kernel(DeviceData dd, const P
   __shared__ uint64_t ip0[WAVES];
   __shared__ uint32_t s0[WAVES];
   __shared__ double gpart[WAVES][6 + NTRI];
+  __shared__ double f64tab[82];  // the FP64 sweep's exp / log tables
   const int q = blockIdx.x;
   const PtmgRequest r = req[q];
   const int k = (int)dd.k;
@@ -172,7 +174,9 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_kernel(DeviceData dd, const P
   W.ysc = ysc;
   W.ip0 = ip0;
   W.s0 = s0;
+  fill_f64_tables(f64tab, (int)threadIdx.x);  // setup() ends with a barrier
   W.setup();
+  const lds_double* tab = (const lds_double*)f64tab;
 
   double beta[NB];
 #pragma unroll
@@ -202,7 +206,7 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_kernel(DeviceData dd, const P
           double xr[NB];
           const double eta = row_eta(cell, xr);
           if (y != 0.0) lin += eta * y;
-          csx += exp(eta);
+          csx += fast_exp(eta, tab);
         },
         [&](int s) {
           const double t = wave_sum(csx);
@@ -240,11 +244,11 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_kernel(DeviceData dd, const P
           }
           double xr[NB];
           const double eta = row_eta(cell, xr);
-          const double e = exp(eta);
+          const double e = fast_exp(eta, tab);
           const double we = ym * e;
           const double w = we + gamma;
-          const double lw = log(w);
-          const double iw = 1.0 / w;
+          const double lw = fast_log(w, tab);
+          const double iw = fast_rcp(w);
           const double g = (gamma + y) * iw;
           gs[0] += we - (gamma + y) * lw;  // total
           gs[1] += lw;                     // slpey
@@ -321,9 +325,9 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_kernel(DeviceData dd, const P
           }
           double xr[NB];
           const double eta = row_eta(cell, xr);
-          const double e = exp(eta);
+          const double e = fast_exp(eta, tab);
           const double w = ym * e + gamma;
-          const double iw = 1.0 / w;
+          const double iw = fast_rcp(w);
           const double g = (gamma + y) * iw;
           const double gte = g * iw * e;
           const double wt = e * ymmd - ym * (g * e) + ym * ym * (gte * e);
