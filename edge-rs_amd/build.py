@@ -26,6 +26,7 @@ SOURCES = [
     "resident.cpp",
     "capi_resident.cpp",
     "driver.cpp",
+    "sc_test.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

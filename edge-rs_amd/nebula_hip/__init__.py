@@ -55,10 +55,11 @@ NoGenesAfterFiltering = _err("NoGenesAfterFiltering", 10)
 InvalidBounds = _err("InvalidBounds", 11)
 InvalidDispersion = _err("InvalidDispersion", 12)
 Gpu = _err("Gpu", 13)
+CoefOutOfRange = _err("CoefOutOfRange", 14)
 _BY_CODE = {c.code: c for c in (EmptyCounts, InvalidArgument, MustBePositive, LengthMismatch,
                                 ShapeMismatch, MalformedSparse, SubjectsNotContiguous,
                                 TooFewSubjects, MissingIntercept, NoGenesAfterFiltering,
-                                InvalidBounds, InvalidDispersion, Gpu)}
+                                InvalidBounds, InvalidDispersion, Gpu, CoefOutOfRange)}
 
 
 def _raise(code, err):
@@ -453,6 +454,41 @@ def bobyqa_device_trace(problem, x0, lower, upper, max_trace=2000):
         _raise(-status - 100, err)
     rows = min(ev.value, max_trace)
     return tr[: rows * (n + 1)].reshape(rows, n + 1), status
+
+
+def glm_sc_test(coefficients, covariance, coef=None, contrast=None):
+    """Wald test of one coefficient (coef) or a contrast (n_coef weights) per
+    gene on a fit's coefficients (genes x n_coef) and packed covariance
+    (genes x n_coef (n_coef + 1) / 2), as NebulaFit holds them
+    (nh_glm_sc_test, glm_sc_test in src/sc/test.rs:98).  Returns a dict of
+    log_fc, se, z, p_value and fdr arrays."""
+    b = np.ascontiguousarray(coefficients, dtype=np.float64)
+    v = np.ascontiguousarray(covariance, dtype=np.float64)
+    n_genes = b.shape[0] if b.ndim == 2 else (1 if b.size else 0)
+    n_coef = b.shape[1] if b.ndim == 2 else b.size
+    c = None
+    cp = None
+    if contrast is not None:
+        c, cp = _d(contrast)
+        if c.size != n_coef:
+            raise LengthMismatch(f"contrast: expected {n_coef}, got {c.size}")
+    elif coef is None:
+        raise InvalidArgument("give coef or contrast")
+    out = {k: np.zeros(n_genes) for k in ("log_fc", "se", "z", "p_value", "fdr")}
+    err = C.create_string_buffer(512)
+    L = lib()
+    L.nh_glm_sc_test.restype = C.c_int
+    L.nh_glm_sc_test.argtypes = [_dp, C.c_size_t, _dp, C.c_size_t, C.c_size_t, C.c_size_t,
+                                 C.c_size_t, _dp, _dp, _dp, _dp, _dp, _dp, C.c_char_p,
+                                 C.c_size_t]
+    rc = L.nh_glm_sc_test(b.ctypes.data_as(_dp), b.size, v.ctypes.data_as(_dp), v.size,
+                          n_genes, n_coef, int(coef or 0), cp,
+                          *[out[k].ctypes.data_as(_dp) for k in ("log_fc", "se", "z",
+                                                                 "p_value", "fdr")],
+                          err, 512)
+    if rc != 0:
+        _raise(rc, err)
+    return out
 
 
 def selftest_f64_math(x, which, device=0):

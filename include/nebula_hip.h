@@ -44,7 +44,8 @@ enum {
   NH_ERR_NO_GENES_AFTER_FILTERING = 10, /* EdgeErrors::NoGenesAfterFiltering */
   NH_ERR_INVALID_BOUNDS = 11,         /* EdgeErrors::InvalidBounds */
   NH_ERR_INVALID_DISPERSION = 12,     /* EdgeErrors::InvalidDispersion */
-  NH_ERR_GPU = 13                     /* EdgeErrors::Gpu */
+  NH_ERR_GPU = 13,                    /* EdgeErrors::Gpu */
+  NH_ERR_COEF_OUT_OF_RANGE = 14       /* EdgeErrors::CoefOutOfRange (glm_sc_test) */
 };
 
 /* NEBULA method, NebulaMethod (src/sc/nebula.rs:195-205). */
@@ -309,6 +310,26 @@ int nh_bobyqa_device_trace(int problem, int n, const double* x0,
  * Returns NH_OK or an NH_ERR_* code. */
 int nh_selftest_f64_math(const double* x, size_t n, int32_t which, int32_t device,
                          double* out, char* err, size_t err_len);
+
+/* ---------------------------------------------------------------------------
+ * Wald tests on a fit: glm_sc_test (src/sc/test.rs:98-201).
+ * ------------------------------------------------------------------------- */
+
+/* Tests coefficient `coef` (contrast NULL) or the contrast c'beta (contrast:
+ * n_coef weights, not all zero; coef ignored) per gene, on a fit's
+ * coefficients (n_genes x n_coef, row-major, coefficients_len values) and
+ * packed upper-triangular covariance ((i, j), i <= j, at j(j+1)/2 + i;
+ * covariance_len values), as nebula_fit_t holds them.  Writes n_genes
+ * values to each of log_fc (c'beta), se (sqrt(c'Vc), full covariance; 0
+ * when c'Vc <= 0), z (log_fc / se, 0 when se is 0), p_value (chi^2 with 1
+ * df upper tail of z^2, 1 when se is 0) and fdr (Benjamini-Hochberg).
+ * Errors: NH_ERR_MUST_BE_POSITIVE (a zero dimension), NH_ERR_LENGTH_MISMATCH,
+ * NH_ERR_COEF_OUT_OF_RANGE, NH_ERR_INVALID_ARGUMENT (an all-zero contrast). */
+int nh_glm_sc_test(const double* coefficients, size_t coefficients_len,
+                   const double* covariance, size_t covariance_len, size_t n_genes,
+                   size_t n_coef, size_t coef, const double* contrast, double* log_fc,
+                   double* se, double* z, double* p_value, double* fdr, char* err,
+                   size_t err_len);
 
 /* ---------------------------------------------------------------------------
  * Warm-start MLP (no counterpart in edge-rs): 12 features -> 64 -> 64 -> 64
