@@ -252,6 +252,31 @@ def main():
             vh = ll(runs[-1]["minimisers"], ip, ix, dv, g2)
             vc = ll(ref["stage_two"][:, 1:3], ip, ix, dv, g2)
             extra[f"dloglik_{a.engine}_vs_cpu_max"] = float(np.max(np.abs(vh - vc) / np.abs(vc)))
+            # every output against the oracle's nebula_sparse on the sample:
+            # stage three after a fresh run of the timed and of the plain engine
+
+            def rel(x, y, floor):
+                x, y = np.asarray(x)[:n_ref], np.asarray(y)
+                return float(np.max(np.abs(x - y) / np.maximum(np.abs(y), floor)))
+
+            errs = {"genes": int(n_ref),
+                    "note": "max relative error over the sample's genes (coefficients "
+                            "against max(|cpu|, 1e-3)); loglik: the profile objective at "
+                            "the minimisers"}
+            for name in dict.fromkeys((a.engine, "plain")):
+                r2 = sess.stage_two(engines[name])
+                fit = sess.finish()
+                assert np.array_equal(fit.gene_index[:n_ref], ref["gene_index"])
+                vh2 = ll(r2["minimisers"], ip, ix, dv, g2)
+                errs[name] = {
+                    "coefficients": rel(fit.coefficients, ref["coefficients"], 1e-3),
+                    "se": rel(fit.se, ref["se"], 1e-12),
+                    "subject_overdispersion": rel(fit.subject_overdispersion,
+                                                  ref["subject_overdispersion"], 1e-12),
+                    "cell_overdispersion": rel(fit.cell_overdispersion,
+                                               ref["cell_overdispersion"], 1e-12),
+                    "loglik": float(np.max(np.abs(vh2 - vc) / np.abs(vc)))}
+            extra["output_errors_vs_cpu"] = errs
     except Exception as e:  # the oracle is optional on a box without its build
         log(f"oracle checks skipped: {e}")
 
