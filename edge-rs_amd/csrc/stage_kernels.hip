@@ -102,30 +102,50 @@ struct PtmgWalk {
     __syncthreads();
   }
 
-  // Calls cell(s, cell, y) for every cell of the wave's block (active lanes
-  // only) and flush(s) after each subject segment, uniformly across the wave.
+  // Calls cell(s, cell, y, row) for every cell of the wave's block (active
+  // lanes only; row = the cell's packed design row, R doubles) and flush(s)
+  // after each subject segment, uniformly across the wave.  The next 64 CSR
+  // entries are held in registers and reloaded as soon as a tile has taken
+  // its own, so their loads overlap the tile's arithmetic; every load is
+  // issued at a clamped in-bounds index through global-address-space pointers.
   template <class Cell, class Flush>
   __device__ void walk(Cell&& cell, Flush&& flush) {
+    constexpr int R = Dims<NB>::R;
     const uint32_t cb = bb(wave), ce = be(wave);
+    if (cb >= ce) return;
+    const gconst_double* rows = (const gconst_double*)dd.row64;
+    const gconst_u32* gcell = (const gconst_u32*)dd.gcell;
+    const gconst_double* gy = (const gconst_double*)dd.gy64;
+    const uint64_t e_last = g1 > 0 ? g1 - 1 : 0;
     uint64_t ip = ip0[wave];
     int s = (int)s0[wave];
-    if (cb >= ce) return;
+    uint64_t e = ip + (uint64_t)lane;
+    uint64_t ec = e < g1 ? e : e_last;
+    uint32_t wc = gcell[ec];
+    double wy = gy[ec];
+    wc = e < g1 ? wc : 0xffffffffu;
     for (; s < k && dd.fid[s] < ce; ++s) {
       const uint32_t lo = max(dd.fid[s], cb), hi = min(dd.fid[s + 1], ce);
       for (uint32_t base = lo; base < hi; base += kWave) {
         const uint32_t lim = min(base + (uint32_t)kWave, hi);
-        uint64_t e = ip + (uint64_t)lane;
-        uint32_t c = e < g1 ? dd.gcell[e] : 0xffffffffu;
-        bool inrow = e < g1 && c < lim;
-        int m = popc64(wave_ballot(inrow));
+        const uint32_t cl = base + (uint32_t)lane;
+        double xrow[R];
+        load_row<double, R>(rows, cl < lim ? cl : lim - 1, xrow);
+        // the window's entries inside [base, lim) form a prefix of it
+        const bool in = wc < lim;
+        const int m = popc64(wave_ballot(in));
         ysc[wave][lane] = 0.0;
         wave_lds_sync();
-        if (inrow) ysc[wave][c - base] = dd.gy64[e];
+        if (in) ysc[wave][wc - base] = wy;
         wave_lds_sync();
         const double y = ysc[wave][lane];
         ip += (uint64_t)m;
-        const uint32_t cl = base + (uint32_t)lane;
-        if (cl < lim) cell(s, cl, y);
+        e = ip + (uint64_t)lane;
+        ec = e < g1 ? e : e_last;
+        wc = gcell[ec];
+        wy = gy[ec];
+        wc = e < g1 ? wc : 0xffffffffu;
+        if (cl < lim) cell(s, cl, y, xrow);
       }
       flush(s);
     }
@@ -187,13 +207,22 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_kernel(DeviceData dd, const P
   const double lambda = alpha / sqrt(exps);
   const double* tot_y = dd.gtot + (size_t)r.gene * k;
 
-  auto row_eta = [&](uint32_t cell, double xr[NB]) {
-    const double* row = dd.x64 + (size_t)cell * NB;
-    double eta = dd.off64[cell];
+  // eta = log offset + x'beta from a packed row (non-intercept columns, then
+  // the log offset), columns added in design order as x64 holds them
+  const int icpt = (int)dd.icpt;
+  auto row_eta = [&](const double* prow, double xr[NB]) {
+    double eta = prow[NB - 1];
+    int q = 0;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
-      xr[j] = row[j];
-      eta += row[j] * beta[j];
+      if (j == icpt) {
+        xr[j] = 1.0;
+        eta += beta[j];
+      } else {
+        xr[j] = prow[q];
+        eta += prow[q] * beta[j];
+        ++q;
+      }
     }
     return eta;
   };
@@ -202,9 +231,9 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_kernel(DeviceData dd, const P
   {
     double csx = 0.0, lin = 0.0;
     W.walk(
-        [&](int, uint32_t cell, double y) {
+        [&](int, uint32_t, double y, const double* prow) {
           double xr[NB];
-          const double eta = row_eta(cell, xr);
+          const double eta = row_eta(prow, xr);
           if (y != 0.0) lin += eta * y;
           csx += fast_exp(eta, tab);
         },
@@ -237,13 +266,13 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_kernel(DeviceData dd, const P
     double ym = 0.0;
     int cur_s = -1;
     W.walk(
-        [&](int s, uint32_t cell, double y) {
+        [&](int s, uint32_t, double y, const double* prow) {
           if (s != cur_s) {
             cur_s = s;
             ym = (tot_y[s] + alpha) / (o[(size_t)s * PS + 0] + lambda);
           }
           double xr[NB];
-          const double eta = row_eta(cell, xr);
+          const double eta = row_eta(prow, xr);
           const double e = fast_exp(eta, tab);
           const double we = ym * e;
           const double w = we + gamma;
@@ -313,7 +342,7 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_kernel(DeviceData dd, const P
     double ym = 0.0, ymmd = 0.0;
     int cur_s = -1;
     W.walk(
-        [&](int s, uint32_t cell, double y) {
+        [&](int s, uint32_t, double y, const double* prow) {
           if (s != cur_s) {
             cur_s = s;
             const double csx = o[(size_t)s * PS + 0];
@@ -324,7 +353,7 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_kernel(DeviceData dd, const P
             ymmd = ymm * (o[(size_t)s * PS + 1] - csx);
           }
           double xr[NB];
-          const double eta = row_eta(cell, xr);
+          const double eta = row_eta(prow, xr);
           const double e = fast_exp(eta, tab);
           const double w = ym * e + gamma;
           const double iw = fast_rcp(w);
