@@ -69,10 +69,12 @@ struct FitKnobs {
   double noise_rel;  // f32 resolution floor relative to |loglik|
   int max_iter;      // Newton budget (50)
   int max_backtrack; // 10
-  // FP32 Newton: take a step whose Newton decrement (predicted
-  // log-likelihood gain) is below the resolution floor without the sweep
-  // that would measure it; the FP64 finish sweeps that point next.  0 keeps
-  // the measured rule.
+  // Stage-two evaluations: stop on the Newton decrement (the step's predicted
+  // log-likelihood gain) where the measured rule would sweep the stepped
+  // point -- in the FP32 loop below the resolution floor, in the FP64 finish
+  // at its first step within eps (log-determinant moved to first order); and
+  // keep an FP32 step whose measured loss is within FP32 rounding.  0 keeps
+  // opt_pml's measured rules (stage three, the request API).
   int decrement_stop;
 };
 
