@@ -12,6 +12,7 @@
 // (:621-636).  Finishing restates finish_gene (:918-1000) and assemble
 // (:2233-2277).
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -21,6 +22,7 @@
 #include <memory>
 #include <numeric>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/nebula_hip.h"
@@ -448,11 +450,25 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
   // zero-count tables would apply (src/sc/zeros.rs:219-262): stage one then
   // runs the quasi-Newton, as edge-rs does.
   {
+    // cells per distinct design row within each subject (rows compared bit
+    // for bit, at most 8 columns)
+    using Row = std::array<uint64_t, 8>;
+    struct RowHash {
+      size_t operator()(const Row& r) const {
+        uint64_t h = 0x9e3779b97f4a7c15ull;
+        for (uint64_t w : r) {
+          h ^= w + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+          h *= 0xbf58476d1ce4e5b9ull;
+        }
+        return (size_t)(h ^ (h >> 31));
+      }
+    };
     size_t tabled = 0;
+    std::unordered_map<Row, size_t, RowHash> groups;
     for (size_t s = 0; s < P.k; ++s) {
-      std::map<std::vector<uint64_t>, size_t> groups;
+      groups.clear();
       for (size_t r = P.fid[s]; r < P.fid[s + 1]; ++r) {
-        std::vector<uint64_t> key(n_coef);
+        Row key{};
         std::memcpy(key.data(), &P.centred[r * n_coef], n_coef * sizeof(double));
         ++groups[key];
       }
@@ -496,20 +512,22 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
     if (pooled) P.kept.push_back(g);
   }
   if (P.kept.empty()) api_fail(NH_ERR_NO_GENES_AFTER_FILTERING, "no genes survived the expression filter");
-  // per kept gene: positive counts, summaries
-  P.gene_ptr.push_back(0);
-  P.genes.resize(P.kept.size());
-  for (size_t idx = 0; idx < P.kept.size(); ++idx) {
+  // per kept gene: positive counts, summaries (genes in parallel, each gene's
+  // sums in its own CSR order), then the kept genes' CSR
+  const size_t KG = P.kept.size();
+  P.genes.resize(KG);
+  std::vector<size_t> npos(KG, 0);
+  parallel_for(KG, 0, [&](size_t idx) {
     const size_t g = P.kept[idx];
     GeneInfo& gi = P.genes[idx];
     double sum = 0.0;
     std::map<double, double> hist;
     gi.xty.assign(n_coef, 0.0);
+    size_t np_ = 0;
     for (uint32_t i = indptr[g]; i < indptr[g + 1]; ++i)
       if (data[i] > 0.0) {
         const double y = data[i];
-        P.cells.push_back(indices[i]);
-        P.counts.push_back(y);
+        ++np_;
         sum += y;
         gi.tail.n_pos += 1.0;
         if (y == 1.0)
@@ -520,14 +538,28 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
           hist[y] += 1.0;
         for (size_t j = 0; j < n_coef; ++j) gi.xty[j] += P.centred[(size_t)indices[i] * n_coef + j] * y;
       }
-    P.gene_ptr.push_back(P.cells.size());
+    npos[idx] = np_;
     for (auto& kv : hist) {
       gi.tail.value.push_back(kv.first);
       gi.tail.mult.push_back(kv.second);
     }
     gi.mean_count = sum / (double)n_cells;
     gi.totals.assign(totals.begin() + g * P.k, totals.begin() + (g + 1) * P.k);
-  }
+  });
+  P.gene_ptr.assign(KG + 1, 0);
+  for (size_t idx = 0; idx < KG; ++idx) P.gene_ptr[idx + 1] = P.gene_ptr[idx] + npos[idx];
+  P.cells.resize(P.gene_ptr[KG]);
+  P.counts.resize(P.gene_ptr[KG]);
+  parallel_for(KG, 0, [&](size_t idx) {
+    const size_t g = P.kept[idx];
+    size_t o = P.gene_ptr[idx];
+    for (uint32_t i = indptr[g]; i < indptr[g + 1]; ++i)
+      if (data[i] > 0.0) {
+        P.cells[o] = indices[i];
+        P.counts[o] = data[i];
+        ++o;
+      }
+  });
   return P;
 }
 
