@@ -589,52 +589,93 @@ void run_stage_one(Prepared& P, Resident& R, int host_threads, uint64_t& rounds)
     tasks.push_back(stage_one_search(start, lo, hi, P.tables, nb, &res[i]));
   }
   const size_t out_stride = ptmg_out_doubles((int)nb, (int)k);
-  const size_t sc_stride = ptmg_scratch_doubles((int)nb, (int)k, 8);
-  DevBuf<PtmgRequest> dreq;
-  DevBuf<double> dparams, dout, dscratch;
-  std::vector<size_t> live;
-  std::vector<PtmgRequest> hreq;
-  std::vector<double> hparams, hout;
-  for (;;) {
-    live.clear();
-    for (size_t i = 0; i < G; ++i)
-      if (tasks[i].waiting()) live.push_back(i);
-    if (live.empty()) break;
-    const size_t n = live.size();
-    hreq.resize(n);
-    hparams.resize(n * np);
-    for (size_t q = 0; q < n; ++q) {
-      auto& pr = tasks[live[q]].h.promise();
-      hreq[q] = PtmgRequest{(uint32_t)live[q], pr.req.hess ? 1u : 0u};
-      std::copy(pr.req.x.begin(), pr.req.x.end(), hparams.begin() + q * np);
+  // Workgroups per request: four per CU over the round's requests, with at
+  // least 8 rows of 64 cells per unit.
+  int dev_cus = 0;
+  NH_HIP(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, R.device));
+  const size_t target_wgs = 4 * (size_t)std::max(1, dev_cus);
+  const size_t max_chunks = std::max<size_t>(1, P.n_cells / (kPtmgWaves * 512));
+
+  // Two groups of genes (alternate kept genes) take turns: while the device
+  // evaluates one group's requests, the host assembles the other group's
+  // results and resumes its searches up to their next requests.  Requests,
+  // points and results stay in mapped host memory that the kernels read and
+  // write in place, so a round puts only kernel launches on the stream.
+  struct Group {
+    std::vector<size_t> genes, live;
+    DevBuf<double> dscratch;
+    PinnedBuf<PtmgRequest> hreq;
+    PinnedBuf<double> hparams, hout{false};
+    hipEvent_t done = nullptr;
+    bool inflight = false;
+    ~Group() {
+      if (done) (void)hipEventDestroy(done);
     }
-    dreq.upload(hreq.data(), n, R.stream);
-    dparams.upload(hparams.data(), n * np, R.stream);
-    dout.alloc(n * out_stride);
-    dscratch.alloc(n * sc_stride);
+  };
+  Group grp[2];
+  for (size_t i = 0; i < G; ++i) grp[i % 2].genes.push_back(i);
+  for (Group& g : grp) NH_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
+  const int threads =
+      host_threads > 0 ? host_threads : (int)std::min(8u, std::max(1u, std::thread::hardware_concurrency()));
+  WorkPool pool(threads - 1);
+
+  auto submit = [&](Group& g) {
+    g.live.clear();
+    for (size_t i : g.genes)
+      if (tasks[i].waiting()) g.live.push_back(i);
+    const size_t n = g.live.size();
+    if (n == 0) return;
+    const size_t chunks = std::min(max_chunks, std::max<size_t>(1, (target_wgs + n - 1) / n));
+    const size_t sc_stride = ptmg_scratch_doubles((int)nb, (int)k, (int)chunks * kPtmgWaves);
+    g.hreq.alloc(n);
+    g.hparams.alloc(n * np);
+    g.hout.alloc(n * out_stride);
+    bool any_hess = false;
+    for (size_t q = 0; q < n; ++q) {
+      auto& pr = tasks[g.live[q]].h.promise();
+      g.hreq.p[q] = PtmgRequest{(uint32_t)g.live[q], pr.req.hess ? 1u : 0u};
+      any_hess = any_hess || pr.req.hess;
+      std::copy(pr.req.x.begin(), pr.req.x.end(), g.hparams.p + q * np);
+    }
+    g.dscratch.alloc(n * sc_stride);
     PtmgLaunch L;
     L.dd = R.view();
-    L.req = dreq.p;
-    L.params = dparams.p;
-    L.out = dout.p;
-    L.scratch = dscratch.p;
+    L.req = g.hreq.dev;
+    L.params = g.hparams.dev;
+    L.out = g.hout.dev;
+    L.scratch = g.dscratch.p;
     L.stride = sc_stride;
     L.n_req = (int)n;
+    L.chunks = (int)chunks;
+    L.any_hess = any_hess;
     L.counters = R.counters.p;
     NH_HIP(launch_ptmg(L, R.stream));
-    hout.resize(n * out_stride);
-    NH_HIP(hipMemcpyAsync(hout.data(), dout.p, n * out_stride * sizeof(double),
-                          hipMemcpyDeviceToHost, R.stream));
-    NH_HIP(hipStreamSynchronize(R.stream));
+    NH_HIP(hipEventRecord(g.done, R.stream));
+    g.inflight = true;
     ++rounds;
-    parallel_for(n, host_threads, [&](size_t q) {
-      auto& pr = tasks[live[q]].h.promise();
+  };
+  auto collect = [&](Group& g) {
+    NH_HIP(hipEventSynchronize(g.done));
+    g.inflight = false;
+    // every search is its own coroutine: assemble and resume in parallel
+    pool.run(g.live.size(), [&](size_t q) {
+      auto& task = tasks[g.live[q]];
+      auto& pr = task.h.promise();
       EvalRes er;
-      assemble_ptmg(P, P.genes[live[q]], &hparams[q * np], &hout[q * out_stride], pr.req.hess,
-                    er.f, er.g, er.h);
+      assemble_ptmg(P, P.genes[g.live[q]], g.hparams.p + q * np, g.hout.p + q * out_stride,
+                    pr.req.hess, er.f, er.g, er.h);
       pr.res = std::move(er);
+      task.h.resume();
     });
-    for (size_t q = 0; q < n; ++q) tasks[live[q]].h.resume();
+  };
+  submit(grp[0]);
+  submit(grp[1]);
+  while (grp[0].inflight || grp[1].inflight) {
+    for (Group& g : grp)
+      if (g.inflight) {
+        collect(g);
+        submit(g);
+      }
   }
   for (size_t i = 0; i < G; ++i) {
     GenePlan& pl = P.plans[i];
@@ -1128,12 +1169,17 @@ int nh_session_autotune(nh_session_t* s, int32_t engine, int32_t* wg_size, doubl
       continue;
     }
     s->tuned_wg = sizes[c];
-    nebula_run_stats_t st;
-    const int rc = nh_session_stage_two(s, engine, nullptr, nullptr, nullptr, &st, err, err_len);
-    if (rc != NH_OK) return rc;
-    if (ms) ms[c] = 1e3 * st.t_stage_two;
-    if (st.t_stage_two < best) {
-      best = st.t_stage_two;
+    // the faster of two runs: one slow run (a host hiccup) does not decide
+    double t = INFINITY;
+    for (int rep = 0; rep < 2; ++rep) {
+      nebula_run_stats_t st;
+      const int rc = nh_session_stage_two(s, engine, nullptr, nullptr, nullptr, &st, err, err_len);
+      if (rc != NH_OK) return rc;
+      t = std::min(t, st.t_stage_two);
+    }
+    if (ms) ms[c] = 1e3 * t;
+    if (t < best) {
+      best = t;
       pick = sizes[c];
     }
   }

@@ -1,9 +1,12 @@
 // Host-side pieces of the driver shared by the stage-two engines.
 #pragma once
 
+#include <atomic>
 #include <cmath>
+#include <condition_variable>
 #include <cstdint>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -17,11 +20,15 @@ struct ApiError {
 };
 [[noreturn]] inline void api_fail(int code, const std::string& m) { throw ApiError{code, m}; }
 
-// Runs f(i) for i in [0, n) on up to `threads` host threads.
-inline void parallel_for(size_t n, int threads, const std::function<void(size_t)>& f) {
+// Runs f(i) for i in [0, n) on up to `threads` host threads (0: one per
+// core), each thread taking at least `grain` items so that short loops do not
+// pay for thread start-up; fewer than 16 items run on the calling thread.
+inline void parallel_for(size_t n, int threads, const std::function<void(size_t)>& f,
+                         size_t grain = 1) {
   if (n == 0) return;
   int nt = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
-  if ((size_t)nt > n) nt = (int)n;
+  const size_t cap = n / std::max<size_t>(1, grain);
+  if ((size_t)nt > cap) nt = (int)cap;
   if (nt <= 1 || n < 16) {
     for (size_t i = 0; i < n; ++i) f(i);
     return;
@@ -33,6 +40,71 @@ inline void parallel_for(size_t n, int threads, const std::function<void(size_t)
     });
   for (auto& th : pool) th.join();
 }
+
+// A fixed set of helper threads for loops run many times over: run(n, f)
+// calls f(i) for every i in [0, n) on the helpers and the calling thread and
+// returns when all are done.  f must not throw.
+class WorkPool {
+ public:
+  explicit WorkPool(int helpers) {
+    for (int t = 0; t < helpers; ++t) threads_.emplace_back([this] { loop(); });
+  }
+  WorkPool(const WorkPool&) = delete;
+  WorkPool& operator=(const WorkPool&) = delete;
+  ~WorkPool() {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      stop_ = true;
+    }
+    wake_.notify_all();
+    for (auto& t : threads_) t.join();
+  }
+  void run(size_t n, const std::function<void(size_t)>& f) {
+    if (threads_.empty() || n < 2) {
+      for (size_t i = 0; i < n; ++i) f(i);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      f_ = &f;
+      n_ = n;
+      next_.store(0);
+      busy_ = threads_.size();
+      ++generation_;
+    }
+    wake_.notify_all();
+    drain();
+    std::unique_lock<std::mutex> lk(m_);
+    idle_.wait(lk, [&] { return busy_ == 0; });
+  }
+
+ private:
+  void drain() {
+    for (size_t i; (i = next_.fetch_add(1)) < n_;) (*f_)(i);
+  }
+  void loop() {
+    uint64_t seen = 0;
+    for (;;) {
+      std::unique_lock<std::mutex> lk(m_);
+      wake_.wait(lk, [&] { return stop_ || generation_ != seen; });
+      if (stop_) return;
+      seen = generation_;
+      lk.unlock();
+      drain();
+      lk.lock();
+      // run() waits for every helper, so none can miss a generation
+      if (--busy_ == 0) idle_.notify_one();
+    }
+  }
+  std::vector<std::thread> threads_;
+  std::mutex m_;
+  std::condition_variable wake_, idle_;
+  const std::function<void(size_t)>* f_ = nullptr;
+  size_t n_ = 0, busy_ = 0;
+  std::atomic<size_t> next_{0};
+  uint64_t generation_ = 0;
+  bool stop_ = false;
+};
 
 // gamma prior of the subject random effect (src/sc/pml.rs:427-429)
 inline void gamma_prior(double exps, double& alpha, double& lambda) {

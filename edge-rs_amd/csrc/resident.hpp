@@ -49,6 +49,38 @@ struct DevBuf {
   }
 };
 
+// Page-locked host buffer (grow-only), mapped into the device's address
+// space: kernels read and write it in place through `dev`, so a small
+// exchange with the host needs no copy on the stream.  Coherent memory
+// (device reads see every host write) suits inputs; non-coherent memory is
+// written through the device's L2 and is visible to the host once the
+// writing kernel has completed, which suits outputs.
+template <class T>
+struct PinnedBuf {
+  T* p = nullptr;    // host address
+  T* dev = nullptr;  // the same memory as the device addresses it
+  size_t n = 0;
+  bool coherent = true;
+  PinnedBuf() = default;
+  explicit PinnedBuf(bool coherent_) : coherent(coherent_) {}
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    if (p) (void)hipHostFree(p);
+    p = dev = nullptr;
+    n = 0;
+    const unsigned flags =
+        hipHostMallocMapped | (coherent ? hipHostMallocCoherent : hipHostMallocNonCoherent);
+    NH_HIP(hipHostMalloc((void**)&p, sizeof(T) * (count ? count : 1), flags));
+    NH_HIP(hipHostGetDevicePointer((void**)&dev, p, 0));
+    n = count;
+  }
+};
+
 struct Resident {
   int device = 0;
   hipStream_t stream = nullptr;

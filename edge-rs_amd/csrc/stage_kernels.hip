@@ -1,10 +1,14 @@
 // Stage one on the device: the sums behind the marginal Poisson-gamma
 // likelihood, its gradient and its Hessian (edge-rs src/sc/ptmg.rs:661-1252:
-// linear_predictor, evaluate, evaluate_hessian / hessian_subject), one
-// workgroup per (gene, point) request, FP64 throughout.
+// linear_predictor, evaluate, evaluate_hessian / hessian_subject) for a batch
+// of (gene, point) requests, FP64 throughout.  Each request's cells are cut
+// into units (one wave each, whole 64-cell rows) spread over `chunks`
+// workgroups, so a round with few requests left still fills the device.
 //
-// Three passes over the gene's cells, each the coalesced 64-cell row walk of
-// pml_fit.hpp with the positive counts merged through LDS:
+// Three passes over the gene's cells, one launch each, each the coalesced
+// 64-cell row walk of pml_fit.hpp with the positive counts merged through LDS;
+// every unit writes its own partial sums, and a pass reads the subject totals
+// the previous launch left:
 //   1. e = exp(eta) per cell: per-subject sum of e, per-gene sum of y * eta
 //      (exp, log and 1/w by the FP64 sweep's table-driven versions);
 //   2. with ym = (Y_s + alpha) / (sum e + lambda) per subject: the gradient
@@ -50,28 +54,62 @@ size_t ptmg_out_doubles(int nb, int k) {
   }
 }
 
-size_t ptmg_scratch_doubles(int nb, int k, int waves) {
-  // wave partials (k + waves) * PS, per-wave gene partials waves * PG
+size_t ptmg_scratch_doubles(int nb, int k, int units) {
+  // per-(unit, subject) partials in slots s + unit: (k + units) * PS, then the
+  // per-unit gene partials: units * PG, then the request's output block as
+  // the reduction assembles it, before one coalesced copy to `out`
   const int ps = 4 + 4 * nb, pg = 6 + nb * (nb + 1) / 2;
-  return (size_t)(k + waves) * ps + (size_t)waves * pg;
+  return (size_t)(k + units) * ps + (size_t)units * pg + ptmg_out_doubles(nb, k);
+}
+
+// Cells per unit: the cell range split into `units` blocks, each a whole
+// number of 64-cell rows.
+__host__ __device__ inline uint32_t ptmg_block(uint32_t n_cells, int units) {
+  const uint32_t per = (n_cells + (uint32_t)units - 1) / (uint32_t)units;
+  return (per + kWave - 1) / kWave * kWave;
+}
+
+// Sum of subject s's unit partials (slot s + u for every unit u that holds
+// some of its cells) at value v, added in unit order with the loads issued
+// four at a time; the pass kernels and the reduction call the same function,
+// so every reader of a subject total sees the same bits.
+__device__ inline double ptmg_subject_total(const DeviceData& dd, const double* part, int stride,
+                                            uint32_t blk, int s, int v) {
+  const uint32_t f0 = dd.fid[s], f1 = dd.fid[s + 1];
+  if (f1 <= f0) return 0.0;
+  const int u0 = (int)(f0 / blk), u1 = (int)((f1 - 1) / blk);
+  const double* p = part + (size_t)s * stride + v;
+  double t = 0.0;
+  int u = u0;
+  for (; u + 3 <= u1; u += 4) {
+    const double a = p[(size_t)u * stride], b = p[(size_t)(u + 1) * stride];
+    const double c = p[(size_t)(u + 2) * stride], d = p[(size_t)(u + 3) * stride];
+    t += a;
+    t += b;
+    t += c;
+    t += d;
+  }
+  for (; u <= u1; ++u) t += p[(size_t)u * stride];
+  return t;
 }
 
 template <int NB, int WAVES>
 struct PtmgWalk {
-  static constexpr int NT = WAVES * kWave;
   const DeviceData& dd;
   uint64_t g0, g1;
   uint32_t blk;
-  int k, wave, lane, tid;
+  int k, unit, wave, lane;
   double (*ysc)[kWave];
   uint64_t* ip0;
   uint32_t* s0;
 
-  __device__ uint32_t bb(int w) const { return min(dd.n_cells, (uint32_t)w * blk); }
-  __device__ uint32_t be(int w) const { return min(dd.n_cells, (uint32_t)(w + 1) * blk); }
+  __device__ uint32_t bb() const { return min(dd.n_cells, (uint32_t)unit * blk); }
+  __device__ uint32_t be() const { return min(dd.n_cells, (uint32_t)(unit + 1) * blk); }
 
+  // First CSR entry at or after the unit's first cell (64-way search over the
+  // gene's positive cells) and the subject that holds that cell.
   __device__ void setup() {
-    const uint32_t cb = bb(wave);
+    const uint32_t cb = bb();
     uint64_t lo = g0, hi = g1;
     while (hi - lo > (uint64_t)kWave) {
       uint64_t step = (hi - lo + kWave - 1) / kWave;
@@ -102,16 +140,17 @@ struct PtmgWalk {
     __syncthreads();
   }
 
-  // Calls cell(s, cell, y, row) for every cell of the wave's block (active
-  // lanes only; row = the cell's packed design row, R doubles) and flush(s)
-  // after each subject segment, uniformly across the wave.  The next 64 CSR
+  // Calls begin(s) before and flush(s) after each subject segment of the
+  // unit's block, uniformly across the wave, and cell(s, cell, y, row) for
+  // every cell of the segment (active lanes only; row = the cell's packed
+  // design row, R doubles).  The next 64 CSR
   // entries are held in registers and reloaded as soon as a tile has taken
   // its own, so their loads overlap the tile's arithmetic; every load is
   // issued at a clamped in-bounds index through global-address-space pointers.
-  template <class Cell, class Flush>
-  __device__ void walk(Cell&& cell, Flush&& flush) {
+  template <class Begin, class Cell, class Flush>
+  __device__ void walk(Begin&& begin, Cell&& cell, Flush&& flush) {
     constexpr int R = Dims<NB>::R;
-    const uint32_t cb = bb(wave), ce = be(wave);
+    const uint32_t cb = bb(), ce = be();
     if (cb >= ce) return;
     const gconst_double* rows = (const gconst_double*)dd.row64;
     const gconst_u32* gcell = (const gconst_u32*)dd.gcell;
@@ -126,6 +165,7 @@ struct PtmgWalk {
     wc = e < g1 ? wc : 0xffffffffu;
     for (; s < k && dd.fid[s] < ce; ++s) {
       const uint32_t lo = max(dd.fid[s], cb), hi = min(dd.fid[s + 1], ce);
+      begin(s);
       for (uint32_t base = lo; base < hi; base += kWave) {
         const uint32_t lim = min(base + (uint32_t)kWave, hi);
         const uint32_t cl = base + (uint32_t)lane;
@@ -150,53 +190,52 @@ struct PtmgWalk {
       flush(s);
     }
   }
-
-  // Sum over a subject's wave partials (slot s + w), waves in order.
-  __device__ double subject_total(const double* part, int stride, int s, int v) const {
-    const uint32_t f0 = dd.fid[s], f1 = dd.fid[s + 1];
-    const int w0 = (int)(f0 / blk), w1 = (int)((f1 - 1) / blk);
-    double t = 0.0;
-    for (int w = w0; w <= w1; ++w) t += part[(size_t)(s + w) * stride + v];
-    return t;
-  }
 };
 
-template <int NB, int WAVES>
-__global__ __launch_bounds__(WAVES * 64) void ptmg_kernel(DeviceData dd, const PtmgRequest* req,
-                                                          const double* params, double* out,
-                                                          double* scratch, size_t sstride,
-                                                          FitCounters* ctr) {
+// One pass over one chunk of a request's cells: workgroup b serves request
+// b / chunks, chunk b % chunks; its waves take units chunk * WAVES + wave.
+// Pass 1 writes the per-(unit, subject) sums of e and the unit's sum of
+// y * eta; pass 2 reads the subject totals of e (every unit of the subject,
+// written by the pass-1 launch) and writes the gradient / Hessian-vector
+// sums; pass 3 (Hessian requests only) reads the totals of e and g e and
+// writes the unit's weighted Gram.
+template <int NB, int WAVES, int PASS>
+__global__ __launch_bounds__(WAVES * 64) void ptmg_pass_kernel(DeviceData dd, const PtmgRequest* req,
+                                                               const double* params,
+                                                               double* scratch, size_t sstride,
+                                                               int chunks) {
   using D = PtmgDims<NB>;
-  constexpr int PS = D::PS, NTRI = D::NTRI;
+  constexpr int PS = D::PS, PG = D::PG, NTRI = D::NTRI;
   __shared__ double ysc[WAVES][kWave];
   __shared__ uint64_t ip0[WAVES];
   __shared__ uint32_t s0[WAVES];
-  __shared__ double gpart[WAVES][6 + NTRI];
   __shared__ double f64tab[82];  // the FP64 sweep's exp / log tables
-  const int q = blockIdx.x;
+  const int q = (int)blockIdx.x / chunks, chunk = (int)blockIdx.x % chunks;
   const PtmgRequest r = req[q];
+  const bool hess = r.hess != 0;
+  if (PASS == 3 && !hess) return;  // uniform over the workgroup
   const int k = (int)dd.k;
+  const int units = chunks * WAVES;
   const double* x = params + (size_t)q * (NB + 2);
-  double* o = out + (size_t)q * ptmg_out_doubles_t<NB>(k);
   double* part = scratch + (size_t)q * sstride;
+  double* gpart = part + (size_t)(k + units) * PS;
 
   PtmgWalk<NB, WAVES> W{dd};
   W.g0 = dd.gptr[r.gene];
   W.g1 = dd.gptr[r.gene + 1];
-  {
-    uint32_t per = (dd.n_cells + WAVES - 1) / WAVES;
-    W.blk = (per + kWave - 1) / kWave * kWave;
-  }
+  W.blk = ptmg_block(dd.n_cells, units);
   W.k = k;
-  W.tid = threadIdx.x;
-  W.wave = threadIdx.x / kWave;
-  W.lane = threadIdx.x % kWave;
+  W.wave = (int)threadIdx.x / kWave;
+  W.lane = (int)threadIdx.x % kWave;
+  W.unit = chunk * WAVES + W.wave;
   W.ysc = ysc;
   W.ip0 = ip0;
   W.s0 = s0;
   fill_f64_tables(f64tab, (int)threadIdx.x);  // setup() ends with a barrier
   W.setup();
   const lds_double* tab = (const lds_double*)f64tab;
+  const int u = W.unit;
+  const bool lead = W.lane == 0;
 
   double beta[NB];
 #pragma unroll
@@ -212,25 +251,26 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_kernel(DeviceData dd, const P
   const int icpt = (int)dd.icpt;
   auto row_eta = [&](const double* prow, double xr[NB]) {
     double eta = prow[NB - 1];
-    int q = 0;
+    int c = 0;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
       if (j == icpt) {
         xr[j] = 1.0;
         eta += beta[j];
       } else {
-        xr[j] = prow[q];
-        eta += prow[q] * beta[j];
-        ++q;
+        xr[j] = prow[c];
+        eta += prow[c] * beta[j];
+        ++c;
       }
     }
     return eta;
   };
 
-  // ---- pass 1: subject sums of e, gene sum of y * eta ----
-  {
+  if (PASS == 1) {
+    // ---- subject sums of e, sum of y * eta ----
     double csx = 0.0, lin = 0.0;
     W.walk(
+        [&](int) {},
         [&](int, uint32_t, double y, const double* prow) {
           double xr[NB];
           const double eta = row_eta(prow, xr);
@@ -239,24 +279,13 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_kernel(DeviceData dd, const P
         },
         [&](int s) {
           const double t = wave_sum(csx);
-          if (W.lane == 0) part[(size_t)(s + W.wave) * PS] = t;
+          if (lead) part[(size_t)(s + u) * PS] = t;
           csx = 0.0;
         });
     const double l = wave_sum(lin);
-    if (W.lane == 0) gpart[W.wave][0] = l;
-    __syncthreads();
-    for (int s = threadIdx.x; s < k; s += WAVES * 64) o[(size_t)s * PS + 0] = W.subject_total(part, PS, s, 0);
-    if (threadIdx.x == 0) {
-      double l2 = 0.0;
-      for (int w = 0; w < WAVES; ++w) l2 += gpart[w][0];
-      o[(size_t)k * PS + 0] = l2;
-    }
-    __syncthreads();
-  }
-
-  // ---- pass 2: gradient (and Hessian vector) sums ----
-  const bool hess = r.hess != 0;
-  {
+    if (lead) gpart[(size_t)u * PG] = l;
+  } else if (PASS == 2) {
+    // ---- gradient (and Hessian vector) sums ----
     double acc[PS - 1];
     double gs[5];
 #pragma unroll
@@ -264,13 +293,11 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_kernel(DeviceData dd, const P
 #pragma unroll
     for (int v = 0; v < 5; ++v) gs[v] = 0.0;
     double ym = 0.0;
-    int cur_s = -1;
     W.walk(
-        [&](int s, uint32_t, double y, const double* prow) {
-          if (s != cur_s) {
-            cur_s = s;
-            ym = (tot_y[s] + alpha) / (o[(size_t)s * PS + 0] + lambda);
-          }
+        [&](int s) {
+          ym = (tot_y[s] + alpha) / (ptmg_subject_total(dd, part, PS, W.blk, s, 0) + lambda);
+        },
+        [&](int, uint32_t, double y, const double* prow) {
           double xr[NB];
           const double eta = row_eta(prow, xr);
           const double e = fast_exp(eta, tab);
@@ -312,46 +339,31 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_kernel(DeviceData dd, const P
 #pragma unroll
           for (int v = 0; v < PS - 1; ++v) {
             const double t = wave_sum(acc[v]);
-            if (W.lane == 0) part[(size_t)(s + W.wave) * PS + 1 + v] = t;
+            if (lead) part[(size_t)(s + u) * PS + 1 + v] = t;
             acc[v] = 0.0;
           }
         });
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
       const double t = wave_sum(gs[v]);
-      if (W.lane == 0) gpart[W.wave][1 + v] = t;
+      if (lead) gpart[(size_t)u * PG + 1 + v] = t;
     }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < k * (PS - 1); idx += WAVES * 64) {
-      const int s = idx / (PS - 1), v = 1 + idx % (PS - 1);
-      o[(size_t)s * PS + v] = W.subject_total(part, PS, s, v);
-    }
-    if (threadIdx.x < 5) {
-      double t = 0.0;
-      for (int w = 0; w < WAVES; ++w) t += gpart[w][1 + threadIdx.x];
-      o[(size_t)k * PS + 1 + threadIdx.x] = t;
-    }
-    __syncthreads();
-  }
-
-  // ---- pass 3: the weighted Gram of the Hessian's beta block ----
-  if (hess) {
+  } else {
+    // ---- the weighted Gram of the Hessian's beta block ----
     double gr[NTRI];
 #pragma unroll
     for (int v = 0; v < NTRI; ++v) gr[v] = 0.0;
     double ym = 0.0, ymmd = 0.0;
-    int cur_s = -1;
     W.walk(
-        [&](int s, uint32_t, double y, const double* prow) {
-          if (s != cur_s) {
-            cur_s = s;
-            const double csx = o[(size_t)s * PS + 0];
-            const double mu = csx + lambda;
-            const double ystar = tot_y[s] + alpha;
-            ym = ystar / mu;
-            const double ymm = ystar / mu / mu;
-            ymmd = ymm * (o[(size_t)s * PS + 1] - csx);
-          }
+        [&](int s) {
+          const double csx = ptmg_subject_total(dd, part, PS, W.blk, s, 0);
+          const double mu = csx + lambda;
+          const double ystar = tot_y[s] + alpha;
+          ym = ystar / mu;
+          const double ymm = ystar / mu / mu;
+          ymmd = ymm * (ptmg_subject_total(dd, part, PS, W.blk, s, 1) - csx);
+        },
+        [&](int, uint32_t, double y, const double* prow) {
           double xr[NB];
           const double eta = row_eta(prow, xr);
           const double e = fast_exp(eta, tab);
@@ -372,31 +384,73 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_kernel(DeviceData dd, const P
 #pragma unroll
     for (int v = 0; v < NTRI; ++v) {
       const double t = wave_sum(gr[v]);
-      if (W.lane == 0) gpart[W.wave][6 + v] = t;
-    }
-    __syncthreads();
-    for (int v = threadIdx.x; v < NTRI; v += WAVES * 64) {
-      double t = 0.0;
-      for (int w = 0; w < WAVES; ++w) t += gpart[w][6 + v];
-      o[(size_t)k * PS + 6 + v] = t;
+      if (lead) gpart[(size_t)u * PG + 6 + v] = t;
     }
   }
+}
+
+// Per request: the subject totals and the gene sums over every unit, in unit
+// order, assembled in device memory and then copied to the request's output
+// block with whole-wave stores (the output may live in host memory).
+template <int NB>
+__global__ __launch_bounds__(256) void ptmg_reduce_kernel(DeviceData dd, const PtmgRequest* req,
+                                                          double* out, double* scratch,
+                                                          size_t sstride, int units,
+                                                          FitCounters* ctr) {
+  using D = PtmgDims<NB>;
+  constexpr int PS = D::PS, PG = D::PG;
+  const int q = (int)blockIdx.x, k = (int)dd.k;
+  const PtmgRequest r = req[q];
+  const bool hess = r.hess != 0;
+  const uint32_t blk = ptmg_block(dd.n_cells, units);
+  const size_t ostride = ptmg_out_doubles_t<NB>(k);
+  double* part = scratch + (size_t)q * sstride;
+  const double* gpart = part + (size_t)(k + units) * PS;
+  double* o = part + (size_t)(k + units) * PS + (size_t)units * PG;
+  const int wave = (int)threadIdx.x / kWave, lane = (int)threadIdx.x % kWave;
+  const int nw = (int)blockDim.x / kWave;
+  for (int idx = (int)threadIdx.x; idx < k * PS; idx += (int)blockDim.x)
+    o[idx] = ptmg_subject_total(dd, part, PS, blk, idx / PS, idx % PS);
+  // gene sums: one wave per value, lanes over the units, then a wave sum
+  const int ng = hess ? PG : 6;
+  for (int v = wave; v < ng; v += nw) {
+    double t = 0.0;
+    for (int w = lane; w < units; w += kWave) t += gpart[(size_t)w * PG + v];
+    t = wave_sum(t);
+    if (lane == 0) o[(size_t)k * PS + v] = t;
+  }
+  __syncthreads();
+  double* dst = out + (size_t)q * ostride;
+  const size_t nout = (size_t)k * PS + (size_t)ng;
+  for (size_t i = threadIdx.x; i < nout; i += blockDim.x) dst[i] = o[i];
   if (threadIdx.x == 0) {
-    atomicAdd(&ctr->sweeps_f64, hess ? 3ull : 2ull);
-    atomicAdd(&ctr->cells_f64, (unsigned long long)dd.n_cells * (hess ? 3ull : 2ull));
-    atomicAdd(&ctr->nnz_f64, (unsigned long long)(W.g1 - W.g0) * (hess ? 3ull : 2ull));
+    const unsigned long long passes = hess ? 3ull : 2ull;
+    atomicAdd(&ctr->sweeps_f64, passes);
+    atomicAdd(&ctr->cells_f64, (unsigned long long)dd.n_cells * passes);
+    atomicAdd(&ctr->nnz_f64, (unsigned long long)(dd.gptr[r.gene + 1] - dd.gptr[r.gene]) * passes);
   }
 }
 
 template <int NB>
 static hipError_t launch_ptmg_nb(const PtmgLaunch& L, hipStream_t st) {
-  hipLaunchKernelGGL((ptmg_kernel<NB, 8>), dim3(L.n_req), dim3(8 * 64), 0, st, L.dd, L.req,
-                     L.params, L.out, L.scratch, L.stride, L.counters);
+  constexpr int W = kPtmgWaves;
+  const dim3 grid((unsigned)(L.n_req * L.chunks)), block(W * 64);
+  hipLaunchKernelGGL((ptmg_pass_kernel<NB, W, 1>), grid, block, 0, st, L.dd, L.req, L.params,
+                     L.scratch, L.stride, L.chunks);
+  hipLaunchKernelGGL((ptmg_pass_kernel<NB, W, 2>), grid, block, 0, st, L.dd, L.req, L.params,
+                     L.scratch, L.stride, L.chunks);
+  if (L.any_hess)
+    hipLaunchKernelGGL((ptmg_pass_kernel<NB, W, 3>), grid, block, 0, st, L.dd, L.req, L.params,
+                       L.scratch, L.stride, L.chunks);
+  hipLaunchKernelGGL((ptmg_reduce_kernel<NB>), dim3((unsigned)L.n_req), dim3(256), 0, st, L.dd,
+                     L.req, L.out, L.scratch, L.stride, L.chunks * W, L.counters);
   return hipGetLastError();
 }
 
 hipError_t launch_ptmg(const PtmgLaunch& L, hipStream_t st) {
   if (L.n_req == 0) return hipSuccess;
+  if (L.chunks < 1 || L.stride < ptmg_scratch_doubles((int)L.dd.nb, (int)L.dd.k, L.chunks * kPtmgWaves))
+    return hipErrorInvalidValue;
   switch (L.dd.nb) {
     case 1: return launch_ptmg_nb<1>(L, st);
     case 2: return launch_ptmg_nb<2>(L, st);

@@ -12,14 +12,22 @@ struct PtmgRequest {
   uint32_t hess;  // 1: also the Hessian sums
 };
 
+// Waves per stage-one workgroup; a request's cells are cut into
+// chunks * kPtmgWaves units, one wave each.
+constexpr int kPtmgWaves = 4;
+
 struct PtmgLaunch {
   DeviceData dd;
+  // req, params and out may be mapped host memory: the kernels read each
+  // request's entries once per workgroup and write out with whole-wave stores
   const PtmgRequest* req;
   const double* params;  // n_req * (nb + 2): [beta, sigma, phi]
   double* out;           // n_req * ptmg_out_doubles(nb, k)
-  double* scratch;
-  size_t stride;
+  double* scratch;       // n_req * stride, device memory
+  size_t stride;         // >= ptmg_scratch_doubles(nb, k, chunks * kPtmgWaves)
   int n_req;
+  int chunks;            // workgroups per request
+  bool any_hess;         // some request asks for the Hessian
   FitCounters* counters;
 };
 
@@ -29,7 +37,7 @@ struct PtmgLaunch {
 // sum y eta, sum (ym e - (gamma + y) ln w), sum ln w, sum g, sum 1/w,
 // sum g/w, Gram[nb (nb + 1) / 2] (row-major upper triangle).
 size_t ptmg_out_doubles(int nb, int k);
-size_t ptmg_scratch_doubles(int nb, int k, int waves);
+size_t ptmg_scratch_doubles(int nb, int k, int units);
 hipError_t launch_ptmg(const PtmgLaunch& L, hipStream_t st);
 
 }  // namespace nh

@@ -601,7 +601,8 @@ class Session:
             _raise(rc, err)
 
     def autotune(self, engine=ENGINE_BATCHED):
-        """Times stage two at 4, 8 and 16 waves per gene and keeps the fastest;
+        """Times stage two at 4, 8 and 16 waves per gene (the faster of two runs
+        each) and keeps the fastest;
         returns (threads per gene, {threads: ms})."""
         ms = np.zeros(3)
         pick = C.c_int32()

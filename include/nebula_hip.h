@@ -201,9 +201,10 @@ int nh_session_set_mlp(nh_session_t* s, const float* params, char* err, size_t e
 int nh_session_stage_two(nh_session_t* s, int32_t engine, double* minimisers,
                          double* objectives, int32_t* evaluations,
                          nebula_run_stats_t* stats, char* err, size_t err_len);
-/* Autotuner: runs stage two with engine once per workgroup size (256, 512,
+/* Autotuner: runs stage two with engine twice per workgroup size (256, 512,
  * 1024 threads per gene, i.e. 4, 8, 16 waves), keeps the fastest for later
- * calls of this session; ms (3 values) gets each size's stage-two time. */
+ * calls of this session; ms (3 values) gets each size's faster stage-two
+ * time (NaN for a size that launches the same kernel as a smaller one). */
 int nh_session_autotune(nh_session_t* s, int32_t engine, int32_t* wg_size, double* ms,
                         char* err, size_t err_len);
 int nh_session_finish(nh_session_t* s, nebula_fit_t* out, double* t_stage_three,
