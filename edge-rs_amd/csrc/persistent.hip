@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cstring>
 #include <cmath>
+#include <memory>
 #include <numeric>
 #include <vector>
 
@@ -307,14 +308,45 @@ hipError_t launch_persistent(const PersistentArgs& A, int waves, int grid, hipSt
 }
 
 // ---------------------------------------------------------------------------
-// Host side: build the search descriptors, order genes longest first, launch
-// once (the launch pair is captured in a hipGraph and replayed), read back.
-void run_stage_two_batched(Prepared& P, Resident& R, const nebula_hip_options_t& o,
-                           nebula_run_stats_t& st, const LearnedStart* learned) {
+// Host side.  Everything a batched stage-two run needs that depends only on
+// the session's resident data, stage one's results and the launch shape --
+// the search descriptors in work order (longest gene first), the count
+// tails, the warm-start features, the device buffers and the captured
+// queue-reset + persistent launch -- is built on the first run of a shape and
+// kept in the session; a run then launches the network (learned engine) and
+// replays the graph.
+struct StageTwoPlan {
+  int waves = 0;
+  bool learned = false;
+  double warm_step = 0.0, warm_xtol = 0.0;
+  size_t n = 0;
+  std::vector<uint32_t> idx_of;  // search -> kept index
+  DevBuf<DevSearch> dsearch;
+  DevBuf<uint32_t> dorder;
+  DevBuf<unsigned> dqueue;
+  DevBuf<double> dtv, dtm, dscr;
+  DevBuf<float> dfeat, dpred;
+  PinnedBuf<DevSearchOut> hout{false};  // the kernel writes each search's result here
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~StageTwoPlan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+static std::shared_ptr<StageTwoPlan> build_stage_two_plan(const Prepared& P, Resident& R,
+                                                          int waves, int grid,
+                                                          const LearnedStart* learned) {
   const size_t G = P.kept.size(), nb = P.n_coef, k = P.k;
   const nebula_params_t& p = P.params;
+  auto plan = std::make_shared<StageTwoPlan>();
+  StageTwoPlan& S = *plan;
+  S.waves = waves;
+  S.learned = learned != nullptr;
+  S.warm_step = learned ? learned->step : 0.0;
+  S.warm_xtol = learned ? learned->xtol_rel : 0.0;
   std::vector<DevSearch> hs;
-  std::vector<uint32_t> idx_of;  // search -> kept index
   std::vector<double> tv, tm;
   for (size_t i = 0; i < G; ++i) {
     if (P.plans[i].refit != 1) continue;
@@ -334,12 +366,10 @@ void run_stage_two_batched(Prepared& P, Resident& R, const nebula_hip_options_t&
     d.n_one = gi.tail.n_one;
     d.n_two = gi.tail.n_two;
     hs.push_back(d);
-    idx_of.push_back((uint32_t)i);
-    P.plans[i].searched = true;
+    S.idx_of.push_back((uint32_t)i);
   }
   const size_t n = hs.size();
-  st.genes_searched = n;
-  if (n == 0) return;
+  S.n = n;
   // longest first: work per evaluation ~ cells + positive counts
   std::vector<uint32_t> order(n);
   std::iota(order.begin(), order.end(), 0u);
@@ -348,46 +378,25 @@ void run_stage_two_batched(Prepared& P, Resident& R, const nebula_hip_options_t&
     const size_t nb2 = P.gene_ptr[hs[b].gene + 1] - P.gene_ptr[hs[b].gene];
     return na > nb2;
   });
-  int dev_cus = 0;
-  NH_HIP(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, R.device));
-  // tile shape: one gene per workgroup; wide workgroups when genes are few
-  // waves per gene: the tuned or requested size; otherwise 16 (a whole CU per
-  // gene) when there are no more genes than CUs, else 8 (two genes per CU
-  // overlap one's serial steps with the other's sweeps)
-  int waves = 8;
-  if (o.wg_size >= 1024)
-    waves = 16;
-  else if (o.wg_size > 0 && o.wg_size <= 256)
-    waves = 4;
-  else if (o.wg_size == 0 && (int)n <= dev_cus)
-    waves = 16;
-  waves = effective_waves((int)nb, waves);
-  const int per_cu = waves == 16 ? 1 : (waves == 8 ? 2 : 4);
-  const int grid = (int)std::min<size_t>(n, (size_t)std::max(1, dev_cus * per_cu));
-  DevBuf<DevSearch> dsearch;
-  DevBuf<uint32_t> dorder;
-  DevBuf<unsigned> dqueue;
-  DevBuf<double> dtv, dtm, dscr;
-  DevBuf<DevSearchOut> dout;
-  dsearch.upload(hs.data(), n, R.stream);
-  dorder.upload(order.data(), n, R.stream);
-  dtv.upload(tv.data(), tv.size(), R.stream);
-  dtm.upload(tm.data(), tm.size(), R.stream);
-  dqueue.alloc(1);
-  dout.alloc(n);
+  S.dsearch.upload(hs.data(), n, R.stream);
+  S.dorder.upload(order.data(), n, R.stream);
+  S.dtv.upload(tv.data(), tv.size(), R.stream);
+  S.dtm.upload(tm.data(), tm.size(), R.stream);
+  S.dqueue.alloc(1);
+  S.hout.alloc(n);
   const size_t stride = persistent_stride((int)nb, (int)k, waves);
   const bool in_lds = stride * sizeof(double) <= kWorkLdsBytes;
-  if (!in_lds) dscr.alloc(stride * (size_t)grid);
+  if (!in_lds) S.dscr.alloc(stride * (size_t)grid);
   PersistentArgs A;
   A.dd = R.view();
-  A.search = dsearch.p;
-  A.order = dorder.p;
+  A.search = S.dsearch.p;
+  A.order = S.dorder.p;
   A.n_search = (int)n;
-  A.queue = dqueue.p;
-  A.tail_val = dtv.p;
-  A.tail_mult = dtm.p;
-  A.out = dout.p;
-  A.scratch = in_lds ? nullptr : dscr.p;
+  A.queue = S.dqueue.p;
+  A.tail_val = S.dtv.p;
+  A.tail_mult = S.dtm.p;
+  A.out = S.hout.dev;
+  A.scratch = in_lds ? nullptr : S.dscr.p;
   A.stride = stride;
   A.work_in_lds = in_lds ? 1 : 0;
   A.knobs.eps = p.eps;
@@ -404,44 +413,79 @@ void run_stage_two_batched(Prepared& P, Resident& R, const nebula_hip_options_t&
   A.pred = nullptr;
   A.warm_step = 0.0;
   A.warm_xtol = 0.0;
-  DevBuf<float> dfeat, dpred;
   if (learned) {
-    // MLP starts for the searched genes, in search order, on the same stream
+    // features of the searched genes in search order; the network's
+    // predictions land in dpred before each replay
     std::vector<float> hf(n * 12);
     for (size_t q = 0; q < n; ++q)
-      std::memcpy(&hf[q * 12], &P.features[(size_t)idx_of[q] * 12], 12 * sizeof(float));
-    dfeat.upload(hf.data(), hf.size(), R.stream);
-    dpred.alloc(n * 2);
-    NH_HIP(launch_mlp_predict(learned->params, dfeat.p, (int)n, dpred.p, R.stream));
-    A.pred = dpred.p;
+      std::memcpy(&hf[q * 12], &P.features[(size_t)S.idx_of[q] * 12], 12 * sizeof(float));
+    S.dfeat.upload(hf.data(), hf.size(), R.stream);
+    S.dpred.alloc(n * 2);
+    A.pred = S.dpred.p;
     A.warm_step = learned->step;
     A.warm_xtol = learned->xtol_rel;
   }
-  // capture the queue reset + persistent launch once, replay it
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  NH_HIP(hipStreamSynchronize(R.stream));
   NH_HIP(hipStreamBeginCapture(R.stream, hipStreamCaptureModeThreadLocal));
   hipError_t le = launch_persistent(A, waves, grid, R.stream);
-  hipError_t ce = hipStreamEndCapture(R.stream, &graph);
+  hipError_t ce = hipStreamEndCapture(R.stream, &S.graph);
   if (le != hipSuccess) throw HipError(std::string("launch_persistent: ") + hipGetErrorString(le));
   NH_HIP(ce);
-  NH_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  NH_HIP(hipGraphInstantiate(&S.exec, S.graph, nullptr, nullptr, 0));
+  return plan;
+}
+
+void run_stage_two_batched(Prepared& P, Resident& R, const nebula_hip_options_t& o,
+                           nebula_run_stats_t& st, const LearnedStart* learned) {
+  const size_t G = P.kept.size(), nb = P.n_coef;
+  size_t n = 0;
+  for (size_t i = 0; i < G; ++i)
+    if (P.plans[i].refit == 1) {
+      P.plans[i].searched = true;
+      ++n;
+    }
+  st.genes_searched = n;
+  if (n == 0) return;
+  int dev_cus = 0;
+  NH_HIP(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, R.device));
+  // tile shape: one gene per workgroup; wide workgroups when genes are few
+  // waves per gene: the tuned or requested size; otherwise 16 (a whole CU per
+  // gene) when there are no more genes than CUs, else 8 (two genes per CU
+  // overlap one's serial steps with the other's sweeps)
+  int waves = 8;
+  if (o.wg_size >= 1024)
+    waves = 16;
+  else if (o.wg_size > 0 && o.wg_size <= 256)
+    waves = 4;
+  else if (o.wg_size == 0 && (int)n <= dev_cus)
+    waves = 16;
+  waves = effective_waves((int)nb, waves);
+  const int per_cu = waves == 16 ? 1 : (waves == 8 ? 2 : 4);
+  const int grid = (int)std::min<size_t>(n, (size_t)std::max(1, dev_cus * per_cu));
+  std::shared_ptr<StageTwoPlan> plan;
+  for (auto& c : R.stage_two_plans)
+    if (c->waves == waves && c->learned == (learned != nullptr) &&
+        (!learned || (c->warm_step == learned->step && c->warm_xtol == learned->xtol_rel)))
+      plan = c;
+  if (!plan) {
+    plan = build_stage_two_plan(P, R, waves, grid, learned);
+    R.stage_two_plans.push_back(plan);
+  }
+  StageTwoPlan& S = *plan;
+  if (S.n != n) throw HipError("stage-two plan out of date");
+  if (learned) NH_HIP(launch_mlp_predict(learned->params, S.dfeat.p, (int)n, S.dpred.p, R.stream));
   NH_HIP(hipEventRecord(R.ev0, R.stream));
-  NH_HIP(hipGraphLaunch(exec, R.stream));
+  NH_HIP(hipGraphLaunch(S.exec, R.stream));
   NH_HIP(hipEventRecord(R.ev1, R.stream));
-  std::vector<DevSearchOut> ho(n);
-  NH_HIP(hipMemcpyAsync(ho.data(), dout.p, n * sizeof(DevSearchOut), hipMemcpyDeviceToHost,
-                        R.stream));
   NH_HIP(hipStreamSynchronize(R.stream));
   float ms = 0.0f;
   NH_HIP(hipEventElapsedTime(&ms, R.ev0, R.ev1));
   R.kernel_ms += ms;
   R.launches += 1;
-  (void)hipGraphExecDestroy(exec);
-  (void)hipGraphDestroy(graph);
   st.rounds = 1;
+  const DevSearchOut* ho = S.hout.p;
   for (size_t q = 0; q < n; ++q) {
-    GenePlan& pl = P.plans[idx_of[q]];
+    GenePlan& pl = P.plans[S.idx_of[q]];
     pl.found = ho[q].found != 0;
     pl.failed = ho[q].failed != 0;
     pl.v[0] = ho[q].x[0];

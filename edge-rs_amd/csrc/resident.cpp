@@ -105,10 +105,8 @@ DeviceData Resident::view() const {
 }
 
 void Resident::reset_counters() {
-  FitCounters z;
-  std::memset(&z, 0, sizeof(z));
-  NH_HIP(hipMemcpyAsync(counters.p, &z, sizeof(z), hipMemcpyHostToDevice, stream));
-  NH_HIP(hipStreamSynchronize(stream));
+  // stream-ordered before the next launch; never captured into a graph
+  NH_HIP(hipMemsetAsync(counters.p, 0, sizeof(FitCounters), stream));
 }
 
 FitCounters Resident::read_counters() {

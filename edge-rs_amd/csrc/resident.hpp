@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -81,6 +82,8 @@ struct PinnedBuf {
   }
 };
 
+struct StageTwoPlan;
+
 struct Resident {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -104,6 +107,8 @@ struct Resident {
   std::vector<double> h_beta, h_lw;
   double kernel_ms = 0.0;
   unsigned long long launches = 0;
+  // batched stage-two runs built so far (one per launch shape), replayed
+  std::vector<std::shared_ptr<StageTwoPlan>> stage_two_plans;
 
   ~Resident();
   void upload(size_t n_cells, size_t nb, size_t k, const double* design, const double* log_offset,

@@ -83,3 +83,29 @@ def test_useless_network_falls_back_to_the_cold_search():
     v_start = _objective(w, g, np.clip(sg, lo, hi))
     assert np.max(np.abs(v_warm[rej] - v_cold[rej]) / np.abs(v_cold[rej])) < 1e-6
     assert np.all(v_warm[~rej] <= v_start[~rej] * (1 - 1e-12) + 1e-9 * np.abs(v_start[~rej]))
+
+
+def test_replayed_runs_repeat_the_first():
+    """A session keeps each launch shape's stage-two plan (descriptors, count
+    tails, features, buffers, the captured launch) and replays it: runs of
+    the learned and the batched engine, interleaved with each other and with
+    the plain engine, return the same minimisers, objectives and evaluation
+    counts every time, and the replayed batched search still agrees with the
+    plain engine's eager one."""
+    w = synth.config(2, seed=61, n_genes=48, n_cells=20_000)
+    s = _session(w, nh.load_weights())
+    try:
+        engines = (nh.ENGINE_LEARNED, nh.ENGINE_BATCHED)
+        first = {e: s.stage_two(e) for e in engines}
+        for _ in range(2):
+            plain = s.stage_two(nh.ENGINE_PLAIN)
+            for e in engines:
+                r = s.stage_two(e)
+                for key in ("minimisers", "objectives", "evaluations"):
+                    np.testing.assert_array_equal(r[key], first[e][key], err_msg=key)
+    finally:
+        s.close()
+    g = np.arange(len(plain["minimisers"]))
+    v_batched = _objective(w, g, first[nh.ENGINE_BATCHED]["minimisers"])
+    v_plain = _objective(w, g, plain["minimisers"])
+    np.testing.assert_allclose(v_batched, v_plain, rtol=1e-9)
