@@ -1211,6 +1211,12 @@ int nh_session_finish(nh_session_t* s, nebula_fit_t* out, double* t_stage_three,
   return NH_OK;
 }
 
+int nh_session_release_plans(nh_session_t* s) {
+  if (!s) return NH_ERR_INVALID_ARGUMENT;
+  s->R.stage_two_plans.clear();
+  return NH_OK;
+}
+
 void nh_session_free(nh_session_t* s) { delete s; }
 
 int nebula_sparse_hip_ex(size_t n_genes, size_t n_cells, const uint32_t* indptr,

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <cmath>
 #include <memory>
@@ -468,8 +469,10 @@ void run_stage_two_batched(Prepared& P, Resident& R, const nebula_hip_options_t&
         (!learned || (c->warm_step == learned->step && c->warm_xtol == learned->xtol_rel)))
       plan = c;
   if (!plan) {
+    const auto t0 = std::chrono::steady_clock::now();
     plan = build_stage_two_plan(P, R, waves, grid, learned);
     R.stage_two_plans.push_back(plan);
+    st.t_plan = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
   StageTwoPlan& S = *plan;
   if (S.n != n) throw HipError("stage-two plan out of date");

@@ -103,7 +103,8 @@ class RunStats(C.Structure):
                 ("sweeps_f64", C.c_uint64), ("cell_sweeps", C.c_uint64),
                 ("nnz_sweeps", C.c_uint64), ("warm_fallbacks", C.c_uint64),
                 ("fit_kernel_ms", C.c_double), ("fit_kernel_launches", C.c_uint64),
-                ("cell_sweeps_f64", C.c_uint64), ("nnz_sweeps_f64", C.c_uint64)]
+                ("cell_sweeps_f64", C.c_uint64), ("nnz_sweeps_f64", C.c_uint64),
+                ("t_plan", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -536,6 +537,8 @@ class Session:
             L.nh_session_finish.argtypes = [C.c_void_p, C.POINTER(_Fit), _dp, C.c_char_p,
                                             C.c_size_t]
             L.nh_session_free.argtypes = [C.c_void_p]
+            L.nh_session_release_plans.restype = C.c_int
+            L.nh_session_release_plans.argtypes = [C.c_void_p]
             L.nh_session_features.restype = C.c_int
             L.nh_session_features.argtypes = [C.c_void_p, _fp]
             L.nh_session_set_mlp.restype = C.c_int
@@ -629,6 +632,13 @@ class Session:
         lib().nh_session_warm_rejected(self._h, rej.ctypes.data_as(C.POINTER(C.c_uint8)))
         return dict(minimisers=mins, objectives=obj, evaluations=ev, stats=st.as_dict(),
                     warm_rejected=rej.astype(bool))
+
+    def release_plans(self):
+        """Drops the kept stage-two plans; the next run of each launch shape
+        stages its searches and captures its launch again."""
+        rc = lib().nh_session_release_plans(self._h)
+        if rc != 0:
+            _raise(rc, C.create_string_buffer(b"null session"))
 
     def finish(self):
         fit = _Fit()

@@ -134,6 +134,9 @@ typedef struct {
   uint64_t fit_kernel_launches;
   uint64_t cell_sweeps_f64;  /* the same over FP64 sweeps */
   uint64_t nnz_sweeps_f64;
+  double t_plan;             /* seconds of t_stage_two spent building the run's plan
+                                (search descriptors, count tails, features, buffers,
+                                the captured launch); 0 when a session replays one */
 } nebula_run_stats_t;
 
 void nebula_params_default(nebula_params_t* params);
@@ -209,6 +212,11 @@ int nh_session_autotune(nh_session_t* s, int32_t engine, int32_t* wg_size, doubl
                         char* err, size_t err_len);
 int nh_session_finish(nh_session_t* s, nebula_fit_t* out, double* t_stage_three,
                       char* err, size_t err_len);
+/* Frees the session's kept stage-two plans (one per launch shape of the
+ * batched and learned engines: device buffers and captured launches); the
+ * next run of a shape builds its plan again.  No counterpart in edge-rs,
+ * whose GPU path stages every run. */
+int nh_session_release_plans(nh_session_t* s);
 void nh_session_free(nh_session_t* s);
 
 /* ---------------------------------------------------------------------------

@@ -90,17 +90,22 @@ def test_replayed_runs_repeat_the_first():
     tails, features, buffers, the captured launch) and replays it: runs of
     the learned and the batched engine, interleaved with each other and with
     the plain engine, return the same minimisers, objectives and evaluation
-    counts every time, and the replayed batched search still agrees with the
-    plain engine's eager one."""
+    counts every time, also after the plans are dropped and built again, and
+    the replayed batched search still agrees with the plain engine's eager
+    one."""
     w = synth.config(2, seed=61, n_genes=48, n_cells=20_000)
     s = _session(w, nh.load_weights())
     try:
         engines = (nh.ENGINE_LEARNED, nh.ENGINE_BATCHED)
         first = {e: s.stage_two(e) for e in engines}
-        for _ in range(2):
+        assert all(first[e]["stats"]["t_plan"] > 0 for e in engines)
+        for rep in range(3):
+            if rep == 2:
+                s.release_plans()  # the next runs build their plans again
             plain = s.stage_two(nh.ENGINE_PLAIN)
             for e in engines:
                 r = s.stage_two(e)
+                assert (r["stats"]["t_plan"] > 0) == (rep == 2)
                 for key in ("minimisers", "objectives", "evaluations"):
                     np.testing.assert_array_equal(r[key], first[e][key], err_msg=key)
     finally:
