@@ -368,18 +368,43 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
                                           std::to_string(n_coef));
   if (!indptr || !subject_id || !design) api_fail(NH_ERR_INVALID_ARGUMENT, "null input");
   if (indptr[0] != 0) api_fail(NH_ERR_MALFORMED_SPARSE, "indptr must start at zero");
-  for (size_t g = 0; g < n_genes; ++g) {
-    if (indptr[g + 1] < indptr[g]) api_fail(NH_ERR_MALFORMED_SPARSE, "indptr must be non-decreasing");
-    for (uint32_t i = indptr[g]; i < indptr[g + 1]; ++i) {
-      if (indices[i] >= n_cells) api_fail(NH_ERR_MALFORMED_SPARSE, "cell index out of range");
-      if (i > indptr[g] && indices[i] <= indices[i - 1])
+  {
+    // genes checked in parallel; the first offending gene in order decides,
+    // structure before values, as a gene-by-gene scan followed by a pass over
+    // the values would report
+    enum : uint8_t { kOk, kIndptr, kRange, kOrder };
+    std::vector<uint8_t> shape(n_genes, kOk), value(n_genes, 0);
+    parallel_for(n_genes, 0, [&](size_t g) {
+      if (indptr[g + 1] < indptr[g]) {
+        shape[g] = kIndptr;
+        return;
+      }
+      for (uint32_t i = indptr[g]; i < indptr[g + 1]; ++i) {
+        if (indices[i] >= n_cells) {
+          shape[g] = kRange;
+          return;
+        }
+        if (i > indptr[g] && indices[i] <= indices[i - 1]) {
+          shape[g] = kOrder;
+          return;
+        }
+      }
+      for (uint32_t i = indptr[g]; i < indptr[g + 1]; ++i)
+        if (!(std::isfinite(data[i]) && data[i] >= 0.0)) {
+          value[g] = 1;
+          return;
+        }
+    }, 16);
+    for (size_t g = 0; g < n_genes; ++g) {
+      if (shape[g] == kIndptr) api_fail(NH_ERR_MALFORMED_SPARSE, "indptr must be non-decreasing");
+      if (shape[g] == kRange) api_fail(NH_ERR_MALFORMED_SPARSE, "cell index out of range");
+      if (shape[g] == kOrder)
         api_fail(NH_ERR_MALFORMED_SPARSE, "cell indices must increase within a gene");
     }
+    for (size_t g = 0; g < n_genes; ++g)
+      if (value[g])
+        api_fail(NH_ERR_INVALID_ARGUMENT, "The counts hold a value that is not a non-negative finite number.");
   }
-  const size_t nnz = indptr[n_genes];
-  for (size_t i = 0; i < nnz; ++i)
-    if (!(std::isfinite(data[i]) && data[i] >= 0.0))
-      api_fail(NH_ERR_INVALID_ARGUMENT, "The counts hold a value that is not a non-negative finite number.");
   Prepared P;
   P.n_genes = n_genes;
   P.n_cells = n_cells;
@@ -463,18 +488,21 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
         return (size_t)(h ^ (h >> 31));
       }
     };
-    size_t tabled = 0;
-    std::unordered_map<Row, size_t, RowHash> groups;
-    for (size_t s = 0; s < P.k; ++s) {
-      groups.clear();
+    // subjects in parallel, each with its own map
+    std::vector<size_t> tabled_s(P.k, 0);
+    parallel_for(P.k, 0, [&](size_t s) {
+      std::unordered_map<Row, size_t, RowHash> groups;
+      groups.reserve(P.fid[s + 1] - P.fid[s]);
       for (size_t r = P.fid[s]; r < P.fid[s + 1]; ++r) {
         Row key{};
         std::memcpy(key.data(), &P.centred[r * n_coef], n_coef * sizeof(double));
         ++groups[key];
       }
       for (auto& kv : groups)
-        if (kv.second >= 32) tabled += kv.second;
-    }
+        if (kv.second >= 32) tabled_s[s] += kv.second;
+    });
+    size_t tabled = 0;
+    for (size_t t : tabled_s) tabled += t;
     P.tables = !((double)tabled < 0.5 * (double)n_cells);
   }
   // cell-level columns (ptmg.rs:1962-2011)
@@ -495,9 +523,10 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
   for (size_t s = 0; s < P.k; ++s)
     for (size_t i = P.fid[s]; i < P.fid[s + 1]; ++i) subject_of[i] = s;
   std::vector<double> totals(n_genes * P.k, 0.0);
-  for (size_t g = 0; g < n_genes; ++g)
+  parallel_for(n_genes, 0, [&](size_t g) {
     for (uint32_t i = indptr[g]; i < indptr[g + 1]; ++i)
       if (data[i] > 0.0) totals[g * P.k + subject_of[indices[i]]] += data[i];
+  }, 16);
   for (size_t g = 0; g < n_genes; ++g) {
     double total = 0.0;
     for (size_t s = 0; s < P.k; ++s) total += totals[g * P.k + s];
@@ -521,7 +550,12 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
     const size_t g = P.kept[idx];
     GeneInfo& gi = P.genes[idx];
     double sum = 0.0;
-    std::map<double, double> hist;
+    // count histogram: whole counts below kDense in an array, any other value
+    // in a map; read back in increasing value
+    constexpr size_t kDense = 4096;
+    std::vector<double> dense(kDense, 0.0);
+    size_t top = 0;
+    std::map<double, double> other;
     gi.xty.assign(n_coef, 0.0);
     size_t np_ = 0;
     for (uint32_t i = indptr[g]; i < indptr[g + 1]; ++i)
@@ -530,19 +564,30 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
         ++np_;
         sum += y;
         gi.tail.n_pos += 1.0;
-        if (y == 1.0)
+        if (y == 1.0) {
           gi.tail.n_one += 1.0;
-        else if (y == 2.0)
+        } else if (y == 2.0) {
           gi.tail.n_two += 1.0;
-        else
-          hist[y] += 1.0;
+        } else if (y < (double)kDense && y == std::floor(y)) {
+          dense[(size_t)y] += 1.0;
+          top = std::max(top, (size_t)y);
+        } else {
+          other[y] += 1.0;
+        }
         for (size_t j = 0; j < n_coef; ++j) gi.xty[j] += P.centred[(size_t)indices[i] * n_coef + j] * y;
       }
     npos[idx] = np_;
-    for (auto& kv : hist) {
-      gi.tail.value.push_back(kv.first);
-      gi.tail.mult.push_back(kv.second);
+    auto it = other.begin();
+    auto put = [&](double v, double m) {
+      gi.tail.value.push_back(v);
+      gi.tail.mult.push_back(m);
+    };
+    for (size_t v = 3; v <= top; ++v) {
+      if (dense[v] == 0.0) continue;
+      for (; it != other.end() && it->first < (double)v; ++it) put(it->first, it->second);
+      put((double)v, dense[v]);
     }
+    for (; it != other.end(); ++it) put(it->first, it->second);
     gi.mean_count = sum / (double)n_cells;
     gi.totals.assign(totals.begin() + g * P.k, totals.begin() + (g + 1) * P.k);
   });

@@ -42,8 +42,8 @@ struct Prepared {
   std::vector<size_t> cell_columns, kept;
   // kept genes' positive counts, CSR over kept genes
   std::vector<uint64_t> gene_ptr;
-  std::vector<uint32_t> cells;
-  std::vector<double> counts;
+  RawVec<uint32_t> cells;
+  RawVec<double> counts;
   std::vector<GeneInfo> genes;
   std::vector<GenePlan> plans;
   std::vector<float> features;  // kept x 12 warm-start features (after stage one)

@@ -9,6 +9,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <utility>
 #include <vector>
 
 namespace nh {
@@ -20,16 +21,22 @@ struct ApiError {
 };
 [[noreturn]] inline void api_fail(int code, const std::string& m) { throw ApiError{code, m}; }
 
-// Runs f(i) for i in [0, n) on up to `threads` host threads (0: one per
-// core), each thread taking at least `grain` items so that short loops do not
-// pay for thread start-up; fewer than 16 items run on the calling thread.
+// Host threads a loop uses when the caller asks for 0 (the default): one per
+// core, at most 16 -- the set-up loops are memory-bound well before that.
+inline int default_threads() {
+  return (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+}
+
+// Runs f(i) for i in [0, n) on up to `threads` host threads (0: the
+// default), each thread taking at least `grain` items so that short loops do
+// not pay for thread start-up.
 inline void parallel_for(size_t n, int threads, const std::function<void(size_t)>& f,
                          size_t grain = 1) {
   if (n == 0) return;
-  int nt = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+  int nt = threads > 0 ? threads : default_threads();
   const size_t cap = n / std::max<size_t>(1, grain);
   if ((size_t)nt > cap) nt = (int)cap;
-  if (nt <= 1 || n < 16) {
+  if (nt <= 1) {
     for (size_t i = 0; i < n; ++i) f(i);
     return;
   }
@@ -40,6 +47,30 @@ inline void parallel_for(size_t n, int threads, const std::function<void(size_t)
     });
   for (auto& th : pool) th.join();
 }
+
+// std::allocator that leaves trivially constructible elements uninitialised
+// on resize, so a large array is first touched (and its pages faulted in) by
+// the threads that fill it rather than by one zeroing pass.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() = default;
+  template <class U>
+  NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+  template <class U>
+  void construct(U* p) noexcept {
+    ::new ((void*)p) U;
+  }
+  template <class U, class... A>
+  void construct(U* p, A&&... a) {
+    ::new ((void*)p) U(std::forward<A>(a)...);
+  }
+};
+template <class T>
+using RawVec = std::vector<T, NoInitAlloc<T>>;
 
 // A fixed set of helper threads for loops run many times over: run(n, f)
 // calls f(i) for every i in [0, n) on the helpers and the calling thread and

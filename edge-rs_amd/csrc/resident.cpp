@@ -66,18 +66,18 @@ void Resident::upload(size_t nc, size_t nb_, size_t k_, const double* design,
   gptr.upload(gene_ptr, n_genes + 1, stream);
   gcell.upload(cells, nnz, stream);
   gy64.upload(counts, nnz, stream);
-  std::vector<float> hy(nnz);
-  for (size_t i = 0; i < nnz; ++i) hy[i] = (float)counts[i];
-  gy32.upload(hy.data(), nnz, stream);
-  // subject totals per gene
+  // f32 counts and subject totals per gene, genes in parallel
+  RawVec<float> hy(nnz);
   std::vector<double> tot(n_genes * k, 0.0);
-  for (size_t g = 0; g < n_genes; ++g) {
+  parallel_for(n_genes, 0, [&](size_t g) {
     size_t s = 0;
     for (uint64_t i = gene_ptr[g]; i < gene_ptr[g + 1]; ++i) {
+      hy[i] = (float)counts[i];
       while (cells[i] >= hf[s + 1]) ++s;
       tot[g * k + s] += counts[i];
     }
-  }
+  }, 16);
+  gy32.upload(hy.data(), nnz, stream);
   gtot.upload(tot.data(), tot.size(), stream);
   counters.alloc(1);
   reset_counters();

@@ -111,7 +111,7 @@ typedef struct {
   const float* mlp_weights; /* packed MLP parameters, see nh_mlp_param_count */
   int32_t wg_size;       /* threads per fit workgroup, 0 = autotuned */
   int32_t device;        /* HIP device ordinal */
-  int32_t host_threads;  /* 0 = hardware concurrency */
+  int32_t host_threads;  /* 0 = hardware concurrency, at most 16 */
   double learned_step;   /* warm search's initial BOBYQA step relative to the
                             predicted point (0 = default 0.01) */
   double learned_xtol;   /* warm search's rhoend / rhobeg (0 = default 3e-2) */

@@ -145,6 +145,32 @@ def test_malformed_csr_rejected():
                                                  format="csc"), subject, design, 3, offset)
 
 
+def test_first_offending_gene_decides_and_structure_precedes_values():
+    """Genes are checked in parallel; the report is the one a gene-by-gene
+    scan of the structure followed by a pass over the values gives: a bad
+    count in the first gene loses to a bad index in a later gene, and of two
+    structure errors the earlier gene's is named."""
+    cs, subject, design, offset = _golden_inputs()
+    ip = cs.indptr
+    data = cs.data.copy()
+    data[ip[0]] = -1.0  # gene 0: a negative count
+    ix = cs.indices.copy()
+    ix[ip[3]] = 10_000  # gene 3: a cell out of range
+    j = ip[5] + 1
+    ix[j] = ix[j - 1]   # gene 5: indices not increasing
+    with pytest.raises(nh.MalformedSparse, match="out of range"):
+        nh.nebula_sparse_hip(nh.CompressedSparse(data, ix, ip, cs.shape), subject, design, 3,
+                             offset)
+    ix2 = cs.indices.copy()
+    ix2[j] = ix2[j - 1]
+    with pytest.raises(nh.MalformedSparse, match="increase"):
+        nh.nebula_sparse_hip(nh.CompressedSparse(data, ix2, ip, cs.shape), subject, design, 3,
+                             offset)
+    with pytest.raises(nh.InvalidArgument, match="non-negative"):
+        nh.nebula_sparse_hip(nh.CompressedSparse(data, cs.indices, ip, cs.shape), subject,
+                             design, 3, offset)
+
+
 def test_invalid_bounds_and_tolerances():
     cs, subject, design, offset = _golden_inputs()
     with pytest.raises(nh.InvalidBounds):
