@@ -191,6 +191,37 @@ def ptmg(design, log_offset, counts, cells, totals, fid, params, hessian=True):
     return v.value, g, h
 
 
+_FG = C.CFUNCTYPE(C.c_double, _dp, _dp, C.c_void_p)
+
+
+def lbfgsb(fg, x0, lower, upper, ftol=1e-12, pgtol=1e-10, max_iter=500):
+    """The stage-one quasi-Newton fallback (projected L-BFGS, Armijo along the
+    projection arc; the role of src/numeric/lbfgsb.rs:876) on fg(x) -> (f, g).
+    Returns (x, f, ok)."""
+    n = len(x0)
+    x0, x0p = _d(x0)
+    lo, lop = _d(lower)
+    hi, hip = _d(upper)
+
+    def cb(xp, gp, _ctx):
+        x = np.ctypeslib.as_array(xp, (n,)).copy()
+        f, g = fg(x)
+        gv = np.ctypeslib.as_array(gp, (n,))
+        gv[:] = g
+        return float(f)
+
+    fn = _FG(cb)
+    out = np.zeros(n)
+    f = C.c_double()
+    L = lib()
+    L.oracle_lbfgsb.restype = C.c_int
+    L.oracle_lbfgsb.argtypes = [C.c_size_t, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int, _FG,
+                                C.c_void_p, _dp, C.POINTER(C.c_double)]
+    rc = L.oracle_lbfgsb(n, x0p, lop, hip, ftol, pgtol, max_iter, fn, None,
+                         out.ctypes.data_as(_dp), C.byref(f))
+    return out, f.value, rc == 0
+
+
 def special(which, x):
     x, xp = _d(np.atleast_1d(x))
     y = np.zeros_like(x)

@@ -22,6 +22,9 @@ SHAPES = {
     "p6_30_subjects": (16, 40_000, 30, 6, "dirichlet2", "generic"),
     "p8_config3": (12, 16_000, 12, 8, "dirichlet2", "config3"),
     "180_subjects_p8": (8, 14_400, 180, 8, "equal", "config3"),
+    # subject-level design only: every subject's rows are one group of 500,
+    # zero tables apply and stage one runs its quasi-Newton search
+    "tables_p2": (16, 5_000, 10, 2, "equal", "config1"),
 }
 
 
