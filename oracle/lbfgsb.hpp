@@ -7,9 +7,12 @@
 // written from the published method (Bertsekas' projected Newton framework
 // with the L-BFGS two-loop recursion on the free variables, Armijo search
 // along the projection arc) rather than the compact-representation Cauchy
-// point.  Its iterates follow no reference trace ("parity unpinned" at the
-// iterate level); what the tests pin is the optimum it reaches, which is what
-// stage one hands on.  Stopping rules match the reference's parameters:
+// point.  Its iterates follow no reference trace (not pinned at the iterate
+// level); what the tests pin is the optimum it reaches, which is what stage
+// one hands on: the reference's known-answer problems for its L-BFGS-B
+// (src/numeric/lbfgsb.rs:1040-1296) at their tolerances, scipy's L-BFGS-B on
+// boxed quadratics, and the R goldens through stage one.  On a badly scaled
+// box it stops within pgtol short of the corner the Cauchy point reaches.  Stopping rules match the reference's parameters:
 // relative objective decrease under ftol, projected-gradient inf-norm under
 // pgtol, max_iter iterations, max_line_search evaluations per search.
 #pragma once

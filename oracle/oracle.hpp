@@ -13,9 +13,9 @@
 //   nebula_sparse end to end                            <- R nebula goldens
 //       tests/data/nebula_*.csv and tests/data/e2e/sc_*
 //   digamma / trigamma / lgamma                         <- scipy.special
-//   L-BFGS-B (stage-one fallback)                       <- closed-form optima;
-//       its iterates follow no reference trace: "parity unpinned" at the
-//       iterate level, pinned only through the stage-one optima it reaches.
+//   quasi-Newton (stage-one fallback)                   <- the reference's
+//       L-BFGS-B known-answer problems (src/numeric/lbfgsb.rs:1040-1296) and
+//       scipy at the optimum; iterates follow no reference trace.
 #pragma once
 
 #include <cstddef>
