@@ -118,6 +118,11 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the workgroup-size autotuner (default heuristic)")
     ap.add_argument("--weights", default=None, help="MLP parameter blob (.npy)")
+    ap.add_argument("--learned-step", type=float, default=0.0,
+                    help="warm search's initial step relative to the predicted point "
+                         "(0: the library default)")
+    ap.add_argument("--learned-xtol", type=float, default=0.0,
+                    help="warm search's rhoend / rhobeg (0: the library default)")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="run only a 256 MiB device-to-device copy (the scale for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -148,7 +153,8 @@ def main():
     weights = nh.load_weights(a.weights) if a.weights else nh.load_weights()
     t0 = time.time()
     sess = nh.Session(counts, w.subject_id, w.design, w.n_coef, w.offset, params,
-                      mlp_weights=weights)
+                      mlp_weights=weights, learned_step=a.learned_step,
+                      learned_xtol=a.learned_xtol)
     log(f"session: setup+upload+stage one {time.time() - t0:.2f}s, kept {sess.n_kept}")
 
     eng = engines[a.engine]
