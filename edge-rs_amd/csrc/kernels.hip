@@ -111,6 +111,19 @@ hipError_t launch_f64_math(const double* x, size_t n, int which, double* out, hi
   return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void widen_kernel(const float* x, size_t n, double* out) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+    out[i] = (double)x[i];
+}
+
+hipError_t launch_widen(const float* x, size_t n, double* out, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  size_t blocks = (n + 255) / 256;
+  blocks = blocks > 8192 ? 8192 : blocks;
+  hipLaunchKernelGGL(widen_kernel, dim3((unsigned)blocks), dim3(256), 0, st, x, n, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_batch(const BatchLaunch& L, hipStream_t st) {
   if (L.n_req == 0) return hipSuccess;
   switch (L.dd.nb) {

@@ -65,19 +65,30 @@ void Resident::upload(size_t nc, size_t nb_, size_t k_, const double* design,
   const size_t nnz = gene_ptr[n_genes];
   gptr.upload(gene_ptr, n_genes + 1, stream);
   gcell.upload(cells, nnz, stream);
-  gy64.upload(counts, nnz, stream);
   // f32 counts and subject totals per gene, genes in parallel
   RawVec<float> hy(nnz);
   std::vector<double> tot(n_genes * k, 0.0);
+  std::vector<uint8_t> inexact(n_genes, 0);
   parallel_for(n_genes, 0, [&](size_t g) {
     size_t s = 0;
     for (uint64_t i = gene_ptr[g]; i < gene_ptr[g + 1]; ++i) {
       hy[i] = (float)counts[i];
+      inexact[g] |= (double)hy[i] != counts[i];
       while (cells[i] >= hf[s + 1]) ++s;
       tot[g * k + s] += counts[i];
     }
   }, 16);
   gy32.upload(hy.data(), nnz, stream);
+  // the f64 counts: widened on the device when every count is an f32 value
+  // (whole counts below 2^24 are), else sent as they are
+  bool exact = true;
+  for (uint8_t b : inexact) exact = exact && !b;
+  if (exact) {
+    gy64.alloc(nnz);
+    NH_HIP(launch_widen(gy32.p, nnz, gy64.p, stream));
+  } else {
+    gy64.upload(counts, nnz, stream);
+  }
   gtot.upload(tot.data(), tot.size(), stream);
   counters.alloc(1);
   reset_counters();

@@ -182,6 +182,31 @@ def test_f64_mode_matches_oracle_iterations():
     assert same >= w.n_genes - 1
 
 
+def test_counts_without_an_f32_value_reach_the_device_in_f64():
+    """Whole counts cross to the device as f32 and are widened there; counts
+    that f32 cannot hold (thirds here) cross in f64.  Either way the f64 fit
+    matches the oracle's log-likelihood to 1e-12, which a count rounded to f32
+    (relative error 3e-8) would miss."""
+    w, d, lo, fid = synthetic_gene_batch(seed=13)
+    for scale in (1.0, 1.0 / 3.0):
+        data = w.data * scale
+        if scale != 1.0:
+            assert np.any(data.astype(np.float32).astype(np.float64) != data)
+        rb = nh.ResidentBatch(d, lo, fid, w.indptr.astype(np.uint64), w.indices, data)
+        genes = np.arange(w.n_genes)
+        beta0 = np.zeros((w.n_genes, w.n_coef))
+        r = rb.solve(genes, 0.3, 1.5, beta0, mode=nh.ResidentBatch.SOLVE_F64)
+        rb.close()
+        for g in genes:
+            a, b = int(w.indptr[g]), int(w.indptr[g + 1])
+            cells = w.indices[a:b].astype(np.uint64)
+            tot = np.zeros(len(fid) - 1)
+            np.add.at(tot, np.searchsorted(fid, cells, side="right") - 1, data[a:b])
+            o = oracle.opt_pml(d, lo, data[a:b], cells, fid, tot, beta0[g], 0.3, 1.5)
+            np.testing.assert_allclose(r["log_likelihood"][g], o["log_likelihood"], rtol=1e-12,
+                                       err_msg=f"scale {scale} gene {g}")
+
+
 ENGINES = {"plain": nh.ENGINE_PLAIN, "batched": nh.ENGINE_BATCHED}
 
 
