@@ -936,7 +936,7 @@ void run_stage_two_plain(Prepared& P, Resident& R, int waves, nebula_run_stats_t
 // Stage three (finish_gene, nebula.rs:918-1000): one f64 opt_pml per gene on
 // the device at the chosen variance components, then the covariance.
 void run_stage_three(Prepared& P, Resident& R, int waves, nebula_fit_t* out) {
-  const size_t G = P.kept.size(), nb = P.n_coef, k = P.k;
+  const size_t G = P.kept.size(), nb = P.n_coef;
   const nebula_params_t& p = P.params;
   std::vector<DevRequest> req(G);
   std::vector<double> bi(G * nb);
