@@ -670,7 +670,10 @@ void run_stage_one(Prepared& P, Resident& R, int host_threads, uint64_t& rounds)
       if (tasks[i].waiting()) g.live.push_back(i);
     const size_t n = g.live.size();
     if (n == 0) return;
-    const size_t chunks = std::min(max_chunks, std::max<size_t>(1, (target_wgs + n - 1) / n));
+    // an even count makes the units per request a multiple of 8, so every
+    // workgroup on a given unit lands on the same XCD
+    size_t chunks = std::min(max_chunks, std::max<size_t>(1, (target_wgs + n - 1) / n));
+    if (chunks % 2 == 1 && chunks + 1 <= max_chunks) ++chunks;
     const size_t sc_stride = ptmg_scratch_doubles((int)nb, (int)k, (int)chunks * kPtmgWaves);
     g.hreq.alloc(n);
     g.hparams.alloc(n * np);

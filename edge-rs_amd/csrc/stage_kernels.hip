@@ -192,8 +192,12 @@ struct PtmgWalk {
   }
 };
 
-// One pass over one chunk of a request's cells: workgroup b serves request
-// b / chunks, chunk b % chunks; its waves take units chunk * WAVES + wave.
+// One pass over one unit of the cells for WAVES requests: workgroup b takes
+// unit b % units of requests (b / units) * WAVES + wave, one per wave, so the
+// waves of a workgroup walk the same design rows (read once from L2 for all
+// of them); with units a multiple of 8, every workgroup on a given unit runs
+// on the same XCD (workgroups go to XCDs round robin), whose L2 then keeps
+// that share of the rows for the whole round.
 // Pass 1 writes the per-(unit, subject) sums of e and the unit's sum of
 // y * eta; pass 2 reads the subject totals of e (every unit of the subject,
 // written by the pass-1 launch) and writes the gradient / Hessian-vector
@@ -203,21 +207,25 @@ template <int NB, int WAVES, int PASS>
 __global__ __launch_bounds__(WAVES * 64) void ptmg_pass_kernel(DeviceData dd, const PtmgRequest* req,
                                                                const double* params,
                                                                double* scratch, size_t sstride,
-                                                               int chunks) {
+                                                               int chunks, int n_req) {
   using D = PtmgDims<NB>;
   constexpr int PS = D::PS, PG = D::PG, NTRI = D::NTRI;
   __shared__ double ysc[WAVES][kWave];
   __shared__ uint64_t ip0[WAVES];
   __shared__ uint32_t s0[WAVES];
   __shared__ double f64tab[82];  // the FP64 sweep's exp / log tables
-  const int q = (int)blockIdx.x / chunks, chunk = (int)blockIdx.x % chunks;
-  const PtmgRequest r = req[q];
-  const bool hess = r.hess != 0;
-  if (PASS == 3 && !hess) return;  // uniform over the workgroup
-  const int k = (int)dd.k;
   const int units = chunks * WAVES;
-  const double* x = params + (size_t)q * (NB + 2);
-  double* part = scratch + (size_t)q * sstride;
+  // wave-uniform: keeps the request's values in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
+  const int q = ((int)blockIdx.x / units) * WAVES + wave;
+  const bool have = q < n_req;
+  const PtmgRequest r = have ? req[q] : PtmgRequest{0u, 0u};
+  const bool hess = r.hess != 0;
+  // a wave without work still takes part in the set-up barrier
+  const bool active = have && (PASS != 3 || hess);
+  const int k = (int)dd.k;
+  const double* x = params + (size_t)(have ? q : 0) * (NB + 2);
+  double* part = scratch + (size_t)(have ? q : 0) * sstride;
   double* gpart = part + (size_t)(k + units) * PS;
 
   PtmgWalk<NB, WAVES> W{dd};
@@ -225,14 +233,15 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_pass_kernel(DeviceData dd, co
   W.g1 = dd.gptr[r.gene + 1];
   W.blk = ptmg_block(dd.n_cells, units);
   W.k = k;
-  W.wave = (int)threadIdx.x / kWave;
+  W.wave = wave;
   W.lane = (int)threadIdx.x % kWave;
-  W.unit = chunk * WAVES + W.wave;
+  W.unit = (int)blockIdx.x % units;
   W.ysc = ysc;
   W.ip0 = ip0;
   W.s0 = s0;
   fill_f64_tables(f64tab, (int)threadIdx.x);  // setup() ends with a barrier
   W.setup();
+  if (!active) return;  // no barrier follows
   const lds_double* tab = (const lds_double*)f64tab;
   const int u = W.unit;
   const bool lead = W.lane == 0;
@@ -434,14 +443,16 @@ __global__ __launch_bounds__(256) void ptmg_reduce_kernel(DeviceData dd, const P
 template <int NB>
 static hipError_t launch_ptmg_nb(const PtmgLaunch& L, hipStream_t st) {
   constexpr int W = kPtmgWaves;
-  const dim3 grid((unsigned)(L.n_req * L.chunks)), block(W * 64);
+  // (requests / W, rounded up) x units workgroups, W requests per workgroup
+  const int groups = (L.n_req + W - 1) / W;
+  const dim3 grid((unsigned)(groups * L.chunks * W)), block(W * 64);
   hipLaunchKernelGGL((ptmg_pass_kernel<NB, W, 1>), grid, block, 0, st, L.dd, L.req, L.params,
-                     L.scratch, L.stride, L.chunks);
+                     L.scratch, L.stride, L.chunks, L.n_req);
   hipLaunchKernelGGL((ptmg_pass_kernel<NB, W, 2>), grid, block, 0, st, L.dd, L.req, L.params,
-                     L.scratch, L.stride, L.chunks);
+                     L.scratch, L.stride, L.chunks, L.n_req);
   if (L.any_hess)
     hipLaunchKernelGGL((ptmg_pass_kernel<NB, W, 3>), grid, block, 0, st, L.dd, L.req, L.params,
-                       L.scratch, L.stride, L.chunks);
+                       L.scratch, L.stride, L.chunks, L.n_req);
   hipLaunchKernelGGL((ptmg_reduce_kernel<NB>), dim3((unsigned)L.n_req), dim3(256), 0, st, L.dd,
                      L.req, L.out, L.scratch, L.stride, L.chunks * W, L.counters);
   return hipGetLastError();

@@ -13,7 +13,8 @@ struct PtmgRequest {
 };
 
 // Waves per stage-one workgroup; a request's cells are cut into
-// chunks * kPtmgWaves units, one wave each.
+// chunks * kPtmgWaves units, one wave each, and a workgroup runs one unit of
+// kPtmgWaves requests (about `chunks` workgroups per request).
 constexpr int kPtmgWaves = 4;
 
 struct PtmgLaunch {
@@ -26,7 +27,7 @@ struct PtmgLaunch {
   double* scratch;       // n_req * stride, device memory
   size_t stride;         // >= ptmg_scratch_doubles(nb, k, chunks * kPtmgWaves)
   int n_req;
-  int chunks;            // workgroups per request
+  int chunks;            // units per request / kPtmgWaves
   bool any_hess;         // some request asks for the Hessian
   FitCounters* counters;
 };
