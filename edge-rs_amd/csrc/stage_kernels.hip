@@ -12,8 +12,10 @@
 //   1. e = exp(eta) per cell: per-subject sum of e, per-gene sum of y * eta
 //      (exp, log and 1/w by the FP64 sweep's table-driven versions);
 //   2. with ym = (Y_s + alpha) / (sum e + lambda) per subject: the gradient
-//      sums (and the Hessian's per-subject vectors when asked);
-//   3. Hessian only: the Gram  sum_i W_i x_i x_i'  with
+//      sums (sum g e, sum x e, sum g x e per subject; the gene's sums of
+//      ym e - (gamma + y) ln w, ln w and g);
+//   3. Hessian only: the Hessian's per-subject vectors (hb2, b2, t1, t2),
+//      the gene's sums of 1/w and g/w, and the Gram  sum_i W_i x_i x_i'  with
 //      W_i = e_i (ymm_s d42c_s - ym_s g_i + ym_s^2 g_i e_i / w_i), the single
 //      subject-weighted combination the three curvature blocks of
 //      hessian_subject enter in (ptmg.rs:1159-1171, 1186-1223).
@@ -294,13 +296,14 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_pass_kernel(DeviceData dd, co
     const double l = wave_sum(lin);
     if (lead) gpart[(size_t)u * PG] = l;
   } else if (PASS == 2) {
-    // ---- gradient (and Hessian vector) sums ----
-    double acc[PS - 1];
-    double gs[5];
+    // ---- gradient sums: sum g e, sum x e, sum g x e per subject ----
+    constexpr int NA = 1 + 2 * NB;
+    double acc[NA];
+    double gs[3];
 #pragma unroll
-    for (int v = 0; v < PS - 1; ++v) acc[v] = 0.0;
+    for (int v = 0; v < NA; ++v) acc[v] = 0.0;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) gs[v] = 0.0;
+    for (int v = 0; v < 3; ++v) gs[v] = 0.0;
     double ym = 0.0;
     W.walk(
         [&](int s) {
@@ -319,47 +322,36 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_pass_kernel(DeviceData dd, co
           gs[1] += lw;                     // slpey
           gs[2] += g;                      // gstar
           acc[0] += g * e;                 // acc
-          if (hess) {
-            const double gt = g * iw;
-            const double gte = gt * e;
-            gs[3] += iw;
-            gs[4] += gt;
-            acc[1] += gte - e * iw;  // hb2
-            acc[2] += gte * e;       // b2
-            const double f1 = gt - iw;
 #pragma unroll
-            for (int a = 0; a < NB; ++a) {
-              const double xe = xr[a] * e;
-              acc[3 + a] += xe;
-              acc[3 + NB + a] += g * xe;
-              acc[3 + 2 * NB + a] += xe * f1;
-              acc[3 + 3 * NB + a] += xe * gte;
-            }
-          } else {
-#pragma unroll
-            for (int a = 0; a < NB; ++a) {
-              const double xe = xr[a] * e;
-              acc[3 + a] += xe;
-              acc[3 + NB + a] += g * xe;
-            }
+          for (int a = 0; a < NB; ++a) {
+            const double xe = xr[a] * e;
+            acc[1 + a] += xe;
+            acc[1 + NB + a] += g * xe;
           }
         },
         [&](int s) {
+          // slots 1 (acc) and 4 .. 4 + 2 NB (xf, d41)
 #pragma unroll
-          for (int v = 0; v < PS - 1; ++v) {
+          for (int v = 0; v < NA; ++v) {
             const double t = wave_sum(acc[v]);
-            if (lead) part[(size_t)(s + u) * PS + 1 + v] = t;
+            if (lead) part[(size_t)(s + u) * PS + (v == 0 ? 1 : 3 + v)] = t;
             acc[v] = 0.0;
           }
         });
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
+    for (int v = 0; v < 3; ++v) {
       const double t = wave_sum(gs[v]);
       if (lead) gpart[(size_t)u * PG + 1 + v] = t;
     }
   } else {
-    // ---- the weighted Gram of the Hessian's beta block ----
+    // ---- the Hessian sums: per subject hb2, b2, t1, t2; the weighted Gram
+    // of the beta block; sum 1/w and sum g/w ----
+    constexpr int NH = 2 + 2 * NB;
+    double hacc[NH];
     double gr[NTRI];
+    double hs[2] = {0.0, 0.0};
+#pragma unroll
+    for (int v = 0; v < NH; ++v) hacc[v] = 0.0;
 #pragma unroll
     for (int v = 0; v < NTRI; ++v) gr[v] = 0.0;
     double ym = 0.0, ymmd = 0.0;
@@ -376,20 +368,43 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_pass_kernel(DeviceData dd, co
           double xr[NB];
           const double eta = row_eta(prow, xr);
           const double e = fast_exp(eta, tab);
-          const double w = ym * e + gamma;
+          const double we = ym * e;
+          const double w = we + gamma;
           const double iw = fast_rcp(w);
           const double g = (gamma + y) * iw;
-          const double gte = g * iw * e;
+          const double gt = g * iw;
+          const double gte = gt * e;
+          hs[0] += iw;
+          hs[1] += gt;
+          hacc[0] += gte - e * iw;  // hb2
+          hacc[1] += gte * e;       // b2
+          const double f1 = gt - iw;
           const double wt = e * ymmd - ym * (g * e) + ym * ym * (gte * e);
           int idx = 0;
 #pragma unroll
           for (int a = 0; a < NB; ++a) {
+            const double xe = xr[a] * e;
+            hacc[2 + a] += xe * f1;        // t1
+            hacc[2 + NB + a] += xe * gte;  // t2
             const double xa = wt * xr[a];
 #pragma unroll
             for (int b = a; b < NB; ++b) gr[idx++] += xa * xr[b];
           }
         },
-        [&](int) {});
+        [&](int s) {
+          // slots 2, 3 (hb2, b2) and 4 + 2 NB .. 4 + 4 NB (t1, t2)
+#pragma unroll
+          for (int v = 0; v < NH; ++v) {
+            const double t = wave_sum(hacc[v]);
+            if (lead) part[(size_t)(s + u) * PS + (v < 2 ? 2 + v : 2 + 2 * NB + v)] = t;
+            hacc[v] = 0.0;
+          }
+        });
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const double t = wave_sum(hs[v]);
+      if (lead) gpart[(size_t)u * PG + 4 + v] = t;
+    }
 #pragma unroll
     for (int v = 0; v < NTRI; ++v) {
       const double t = wave_sum(gr[v]);
@@ -418,13 +433,18 @@ __global__ __launch_bounds__(256) void ptmg_reduce_kernel(DeviceData dd, const P
   double* o = part + (size_t)(k + units) * PS + (size_t)units * PG;
   const int wave = (int)threadIdx.x / kWave, lane = (int)threadIdx.x % kWave;
   const int nw = (int)blockDim.x / kWave;
-  for (int idx = (int)threadIdx.x; idx < k * PS; idx += (int)blockDim.x)
-    o[idx] = ptmg_subject_total(dd, part, PS, blk, idx / PS, idx % PS);
+  for (int idx = (int)threadIdx.x; idx < k * PS; idx += (int)blockDim.x) {
+    const int v = idx % PS;
+    // hb2, b2, t1, t2 come from pass 3, which gradient-only requests skip
+    const bool hess_only = v == 2 || v == 3 || v >= 4 + 2 * NB;
+    o[idx] = (hess || !hess_only) ? ptmg_subject_total(dd, part, PS, blk, idx / PS, v) : 0.0;
+  }
   // gene sums: one wave per value, lanes over the units, then a wave sum
   const int ng = hess ? PG : 6;
   for (int v = wave; v < ng; v += nw) {
     double t = 0.0;
-    for (int w = lane; w < units; w += kWave) t += gpart[(size_t)w * PG + v];
+    if (hess || v < 4)  // sum 1/w and sum g/w come from pass 3
+      for (int w = lane; w < units; w += kWave) t += gpart[(size_t)w * PG + v];
     t = wave_sum(t);
     if (lane == 0) o[(size_t)k * PS + v] = t;
   }
