@@ -368,10 +368,21 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
                                           std::to_string(n_coef));
   if (!indptr || !subject_id || !design) api_fail(NH_ERR_INVALID_ARGUMENT, "null input");
   if (indptr[0] != 0) api_fail(NH_ERR_MALFORMED_SPARSE, "indptr must start at zero");
+  // Cells' subject runs (checked for contiguity further down, after the
+  // counts, as before), then one pass over the counts, genes in parallel:
+  // structure and values checked, and for a valid gene its subject totals
+  // and positive count.  The first offending gene in order decides, structure
+  // before values, as a gene-by-gene scan followed by a pass over the values
+  // would report.
+  std::vector<uint32_t> run_of(n_cells, 0);
+  size_t n_runs = 1;
+  for (size_t i = 1; i < n_cells; ++i) {
+    if (subject_id[i] != subject_id[i - 1]) ++n_runs;
+    run_of[i] = (uint32_t)(n_runs - 1);
+  }
+  std::vector<double> totals(n_genes * n_runs, 0.0);
+  std::vector<size_t> positives(n_genes, 0);
   {
-    // genes checked in parallel; the first offending gene in order decides,
-    // structure before values, as a gene-by-gene scan followed by a pass over
-    // the values would report
     enum : uint8_t { kOk, kIndptr, kRange, kOrder };
     std::vector<uint8_t> shape(n_genes, kOk), value(n_genes, 0);
     parallel_for(n_genes, 0, [&](size_t g) {
@@ -379,6 +390,7 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
         shape[g] = kIndptr;
         return;
       }
+      size_t np_ = 0;
       for (uint32_t i = indptr[g]; i < indptr[g + 1]; ++i) {
         if (indices[i] >= n_cells) {
           shape[g] = kRange;
@@ -388,12 +400,14 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
           shape[g] = kOrder;
           return;
         }
-      }
-      for (uint32_t i = indptr[g]; i < indptr[g + 1]; ++i)
-        if (!(std::isfinite(data[i]) && data[i] >= 0.0)) {
-          value[g] = 1;
-          return;
+        const double y = data[i];
+        if (!(std::isfinite(y) && y >= 0.0)) value[g] = 1;
+        if (y > 0.0 && !value[g]) {
+          totals[g * n_runs + run_of[indices[i]]] += y;
+          ++np_;
         }
+      }
+      positives[g] = np_;
     }, 16);
     for (size_t g = 0; g < n_genes; ++g) {
       if (shape[g] == kIndptr) api_fail(NH_ERR_MALFORMED_SPARSE, "indptr must be non-decreasing");
@@ -423,6 +437,7 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
     P.fid.push_back(n_cells);
   }
   P.k = P.fid.size() - 1;
+  if (P.k != n_runs) api_fail(NH_ERR_INVALID_ARGUMENT, "internal: subject runs disagree");
   if (P.k < 2) api_fail(NH_ERR_TOO_FEW_SUBJECTS, "need at least 2 subjects, got " + std::to_string(P.k));
   P.log_offset.assign(n_cells, 0.0);
   double mean_offset = 1.0, cv = 0.0;
@@ -439,28 +454,42 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
   }
   P.log_mean_offset = std::log(mean_offset);
   P.cv2 = cv * cv;
+  // centre_design, row-major passes over the design (each column's sums
+  // still run over the cells in order)
   P.centred.assign(n_cells * n_coef, 0.0);
   P.sds.assign(n_coef, 0.0);
-  for (size_t j = 0; j < n_coef; ++j) {
-    double mean = 0.0;
-    for (size_t i = 0; i < n_cells; ++i) mean += design[i * n_coef + j];
-    mean /= (double)n_cells;
-    double var = 0.0;
-    for (size_t i = 0; i < n_cells; ++i) {
-      double d = design[i * n_coef + j] - mean;
-      P.centred[i * n_coef + j] = d;
-      var += d * d;
+  {
+    std::vector<double> mean(n_coef, 0.0), var(n_coef, 0.0), sd(n_coef, 0.0);
+    for (size_t i = 0; i < n_cells; ++i)
+      for (size_t j = 0; j < n_coef; ++j) mean[j] += design[i * n_coef + j];
+    for (size_t j = 0; j < n_coef; ++j) mean[j] /= (double)n_cells;
+    for (size_t i = 0; i < n_cells; ++i)
+      for (size_t j = 0; j < n_coef; ++j) {
+        const double d = design[i * n_coef + j] - mean[j];
+        P.centred[i * n_coef + j] = d;
+        var[j] += d * d;
+      }
+    // per column: 0 scale by the sd, 1 the intercept (all ones), 2 a zero column
+    std::vector<int> mode(n_coef, 0);
+    for (size_t j = 0; j < n_coef; ++j) {
+      sd[j] = std::sqrt(var[j] / (double)n_cells);
+      if (sd[j] > 0.0) {
+        P.sds[j] = sd[j];
+      } else if (design[j] != 0.0) {
+        mode[j] = 1;
+        P.sds[j] = sd[j];
+      } else {
+        mode[j] = 2;
+        P.sds[j] = -1.0;
+      }
     }
-    double sd = std::sqrt(var / (double)n_cells);
-    if (sd > 0.0) {
-      for (size_t i = 0; i < n_cells; ++i) P.centred[i * n_coef + j] /= sd;
-      P.sds[j] = sd;
-    } else if (design[j] != 0.0) {
-      for (size_t i = 0; i < n_cells; ++i) P.centred[i * n_coef + j] = 1.0;
-      P.sds[j] = sd;
-    } else {
-      P.sds[j] = -1.0;
-    }
+    for (size_t i = 0; i < n_cells; ++i)
+      for (size_t j = 0; j < n_coef; ++j) {
+        if (mode[j] == 0)
+          P.centred[i * n_coef + j] /= sd[j];
+        else if (mode[j] == 1)
+          P.centred[i * n_coef + j] = 1.0;
+      }
   }
   for (double s : P.sds)
     if (s < 0.0) api_fail(NH_ERR_INVALID_ARGUMENT, "Some predictors are a zero vector.");
@@ -518,15 +547,8 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
     }
     if (varies) P.cell_columns.push_back(j);
   }
-  // totals, filter
-  std::vector<size_t> subject_of(n_cells);
-  for (size_t s = 0; s < P.k; ++s)
-    for (size_t i = P.fid[s]; i < P.fid[s + 1]; ++i) subject_of[i] = s;
-  std::vector<double> totals(n_genes * P.k, 0.0);
-  parallel_for(n_genes, 0, [&](size_t g) {
-    for (uint32_t i = indptr[g]; i < indptr[g + 1]; ++i)
-      if (data[i] > 0.0) totals[g * P.k + subject_of[indices[i]]] += data[i];
-  }, 16);
+  // filter, on the subject totals of the first pass (the subjects are the
+  // contiguous runs it used)
   for (size_t g = 0; g < n_genes; ++g) {
     double total = 0.0;
     for (size_t s = 0; s < P.k; ++s) total += totals[g * P.k + s];
@@ -541,11 +563,17 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
     if (pooled) P.kept.push_back(g);
   }
   if (P.kept.empty()) api_fail(NH_ERR_NO_GENES_AFTER_FILTERING, "no genes survived the expression filter");
-  // per kept gene: positive counts, summaries (genes in parallel, each gene's
-  // sums in its own CSR order), then the kept genes' CSR
+  // per kept gene, one more pass over its counts (genes in parallel, each
+  // gene's sums in its own CSR order): summaries, count histogram and the
+  // kept genes' CSR of positive counts, first touched by the threads that
+  // fill it
   const size_t KG = P.kept.size();
   P.genes.resize(KG);
-  std::vector<size_t> npos(KG, 0);
+  P.gene_ptr.assign(KG + 1, 0);
+  for (size_t idx = 0; idx < KG; ++idx)
+    P.gene_ptr[idx + 1] = P.gene_ptr[idx] + positives[P.kept[idx]];
+  P.cells.resize(P.gene_ptr[KG]);
+  P.counts.resize(P.gene_ptr[KG]);
   parallel_for(KG, 0, [&](size_t idx) {
     const size_t g = P.kept[idx];
     GeneInfo& gi = P.genes[idx];
@@ -556,12 +584,13 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
     std::vector<double> dense(kDense, 0.0);
     size_t top = 0;
     std::map<double, double> other;
-    gi.xty.assign(n_coef, 0.0);
-    size_t np_ = 0;
+    size_t o = P.gene_ptr[idx];
     for (uint32_t i = indptr[g]; i < indptr[g + 1]; ++i)
       if (data[i] > 0.0) {
         const double y = data[i];
-        ++np_;
+        P.cells[o] = indices[i];
+        P.counts[o] = y;
+        ++o;
         sum += y;
         gi.tail.n_pos += 1.0;
         if (y == 1.0) {
@@ -574,9 +603,7 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
         } else {
           other[y] += 1.0;
         }
-        for (size_t j = 0; j < n_coef; ++j) gi.xty[j] += P.centred[(size_t)indices[i] * n_coef + j] * y;
       }
-    npos[idx] = np_;
     auto it = other.begin();
     auto put = [&](double v, double m) {
       gi.tail.value.push_back(v);
@@ -590,20 +617,6 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
     for (; it != other.end(); ++it) put(it->first, it->second);
     gi.mean_count = sum / (double)n_cells;
     gi.totals.assign(totals.begin() + g * P.k, totals.begin() + (g + 1) * P.k);
-  });
-  P.gene_ptr.assign(KG + 1, 0);
-  for (size_t idx = 0; idx < KG; ++idx) P.gene_ptr[idx + 1] = P.gene_ptr[idx] + npos[idx];
-  P.cells.resize(P.gene_ptr[KG]);
-  P.counts.resize(P.gene_ptr[KG]);
-  parallel_for(KG, 0, [&](size_t idx) {
-    const size_t g = P.kept[idx];
-    size_t o = P.gene_ptr[idx];
-    for (uint32_t i = indptr[g]; i < indptr[g + 1]; ++i)
-      if (data[i] > 0.0) {
-        P.cells[o] = indices[i];
-        P.counts[o] = data[i];
-        ++o;
-      }
   });
   return P;
 }
@@ -632,6 +645,18 @@ void run_stage_one(Prepared& P, Resident& R, int host_threads, uint64_t& rounds)
     pl.beta_start.assign(nb, 0.0);
     pl.beta_start[P.intercept] = lmc - P.log_mean_offset;
     tasks.push_back(stage_one_search(start, lo, hi, P.tables, nb, &res[i]));
+  }
+  {
+    // each gene's x'y over its positive counts, on the device
+    DevBuf<double> dx;
+    dx.alloc(G * nb);
+    NH_HIP(launch_xty(R.view(), G, dx.p, R.stream));
+    std::vector<double> hx(G * nb);
+    if (G) NH_HIP(hipMemcpyAsync(hx.data(), dx.p, G * nb * sizeof(double), hipMemcpyDeviceToHost,
+                                 R.stream));
+    NH_HIP(hipStreamSynchronize(R.stream));
+    for (size_t i = 0; i < G; ++i)
+      P.genes[i].xty.assign(hx.begin() + i * nb, hx.begin() + (i + 1) * nb);
   }
   const size_t out_stride = ptmg_out_doubles((int)nb, (int)k);
   // Workgroups per request: four per CU over the round's requests, with at

@@ -14,7 +14,8 @@ struct Resident;
 struct GeneInfo {
   double mean_count = 0.0;     // nebula's mct
   std::vector<double> totals;  // per-subject count totals
-  std::vector<double> xty;     // sum over positive cells of x * y (centred design)
+  std::vector<double> xty;     // sum over positive cells of x * y (centred design;
+                               // filled on the device at the start of stage one)
   CountTail tail;              // lgamma tail histogram and n1 / n2
 };
 

@@ -1,8 +1,12 @@
 // Host-side pieces of the driver shared by the stage-two engines.
 #pragma once
 
+#include <sys/mman.h>
+
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
+#include <new>
 #include <condition_variable>
 #include <cstdint>
 #include <functional>
@@ -50,9 +54,12 @@ inline void parallel_for(size_t n, int threads, const std::function<void(size_t)
 
 // std::allocator that leaves trivially constructible elements uninitialised
 // on resize, so a large array is first touched (and its pages faulted in) by
-// the threads that fill it rather than by one zeroing pass.
+// the threads that fill it rather than by one zeroing pass; arrays of 32 MiB
+// and more are 2 MiB aligned and asked for transparent huge pages, so that
+// filling them takes one fault per 2 MiB instead of one per 4 KiB.
 template <class T>
 struct NoInitAlloc : std::allocator<T> {
+  static constexpr size_t kHuge = size_t(2) << 20, kHugeMin = size_t(32) << 20;
   template <class U>
   struct rebind {
     using other = NoInitAlloc<U>;
@@ -60,6 +67,21 @@ struct NoInitAlloc : std::allocator<T> {
   NoInitAlloc() = default;
   template <class U>
   NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+  T* allocate(size_t n) {
+    const size_t bytes = n * sizeof(T);
+    if (bytes < kHugeMin) return std::allocator<T>::allocate(n);
+    const size_t rounded = (bytes + kHuge - 1) / kHuge * kHuge;
+    void* p = std::aligned_alloc(kHuge, rounded);
+    if (!p) throw std::bad_alloc();
+    (void)madvise(p, rounded, MADV_HUGEPAGE);  // advisory: ignored where unsupported
+    return static_cast<T*>(p);
+  }
+  void deallocate(T* p, size_t n) noexcept {
+    if (n * sizeof(T) < kHugeMin)
+      std::allocator<T>::deallocate(p, n);
+    else
+      std::free(p);
+  }
   template <class U>
   void construct(U* p) noexcept {
     ::new ((void*)p) U;

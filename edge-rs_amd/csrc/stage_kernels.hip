@@ -460,6 +460,55 @@ __global__ __launch_bounds__(256) void ptmg_reduce_kernel(DeviceData dd, const P
   }
 }
 
+// Per gene: the sum over its positive counts of y times the centred design
+// row (the constant term of stage one's beta gradient, ptmg.rs's x'y), one
+// workgroup per gene; threads stride over the gene's counts and the block
+// adds the partials in a fixed order.
+template <int NB>
+__global__ __launch_bounds__(256) void xty_kernel(DeviceData dd, double* out) {
+  const uint32_t g = blockIdx.x;
+  const uint64_t a = dd.gptr[g], b = dd.gptr[g + 1];
+  double acc[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) acc[j] = 0.0;
+  for (uint64_t e = a + threadIdx.x; e < b; e += 256) {
+    const double y = dd.gy64[e];
+    const double* x = dd.x64 + (size_t)dd.gcell[e] * NB;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc[j] += x[j] * y;
+  }
+  __shared__ double red[NB][4];
+  const int wave = (int)threadIdx.x / kWave, lane = (int)threadIdx.x % kWave;
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const double t = wave_sum(acc[j]);
+    if (lane == 0) red[j][wave] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < NB) {
+    double t = 0.0;
+    for (int w = 0; w < 4; ++w) t += red[threadIdx.x][w];
+    out[(size_t)g * NB + threadIdx.x] = t;
+  }
+}
+
+hipError_t launch_xty(const DeviceData& dd, size_t n_genes, double* out, hipStream_t st) {
+  if (n_genes == 0) return hipSuccess;
+  const dim3 grid((unsigned)n_genes), block(256);
+  switch (dd.nb) {
+    case 1: hipLaunchKernelGGL(xty_kernel<1>, grid, block, 0, st, dd, out); break;
+    case 2: hipLaunchKernelGGL(xty_kernel<2>, grid, block, 0, st, dd, out); break;
+    case 3: hipLaunchKernelGGL(xty_kernel<3>, grid, block, 0, st, dd, out); break;
+    case 4: hipLaunchKernelGGL(xty_kernel<4>, grid, block, 0, st, dd, out); break;
+    case 5: hipLaunchKernelGGL(xty_kernel<5>, grid, block, 0, st, dd, out); break;
+    case 6: hipLaunchKernelGGL(xty_kernel<6>, grid, block, 0, st, dd, out); break;
+    case 7: hipLaunchKernelGGL(xty_kernel<7>, grid, block, 0, st, dd, out); break;
+    case 8: hipLaunchKernelGGL(xty_kernel<8>, grid, block, 0, st, dd, out); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 template <int NB>
 static hipError_t launch_ptmg_nb(const PtmgLaunch& L, hipStream_t st) {
   constexpr int W = kPtmgWaves;

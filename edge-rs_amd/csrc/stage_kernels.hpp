@@ -40,5 +40,7 @@ struct PtmgLaunch {
 size_t ptmg_out_doubles(int nb, int k);
 size_t ptmg_scratch_doubles(int nb, int k, int units);
 hipError_t launch_ptmg(const PtmgLaunch& L, hipStream_t st);
+// out[g * nb + j] = sum over gene g's positive counts of y * x64[cell * nb + j]
+hipError_t launch_xty(const DeviceData& dd, size_t n_genes, double* out, hipStream_t st);
 
 }  // namespace nh
