@@ -585,6 +585,9 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
     size_t top = 0;
     std::map<double, double> other;
     size_t o = P.gene_ptr[idx];
+    // counters in registers, stored once: neighbouring genes' records share
+    // cache lines and run on different threads
+    double n_pos = 0.0, n_one = 0.0, n_two = 0.0;
     for (uint32_t i = indptr[g]; i < indptr[g + 1]; ++i)
       if (data[i] > 0.0) {
         const double y = data[i];
@@ -592,11 +595,11 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
         P.counts[o] = y;
         ++o;
         sum += y;
-        gi.tail.n_pos += 1.0;
+        n_pos += 1.0;
         if (y == 1.0) {
-          gi.tail.n_one += 1.0;
+          n_one += 1.0;
         } else if (y == 2.0) {
-          gi.tail.n_two += 1.0;
+          n_two += 1.0;
         } else if (y < (double)kDense && y == std::floor(y)) {
           dense[(size_t)y] += 1.0;
           top = std::max(top, (size_t)y);
@@ -615,6 +618,9 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
       put((double)v, dense[v]);
     }
     for (; it != other.end(); ++it) put(it->first, it->second);
+    gi.tail.n_pos = n_pos;
+    gi.tail.n_one = n_one;
+    gi.tail.n_two = n_two;
     gi.mean_count = sum / (double)n_cells;
     gi.totals.assign(totals.begin() + g * P.k, totals.begin() + (g + 1) * P.k);
   });

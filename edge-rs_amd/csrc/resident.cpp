@@ -69,14 +69,25 @@ void Resident::upload(size_t nc, size_t nb_, size_t k_, const double* design,
   RawVec<float> hy(nnz);
   std::vector<double> tot(n_genes * k, 0.0);
   std::vector<uint8_t> inexact(n_genes, 0);
+  // (per-gene flags and sums kept in registers: neighbouring genes run on
+  // different threads and would share cache lines)
   parallel_for(n_genes, 0, [&](size_t g) {
     size_t s = 0;
+    bool inex = false;
+    double t = 0.0;
     for (uint64_t i = gene_ptr[g]; i < gene_ptr[g + 1]; ++i) {
-      hy[i] = (float)counts[i];
-      inexact[g] |= (double)hy[i] != counts[i];
-      while (cells[i] >= hf[s + 1]) ++s;
-      tot[g * k + s] += counts[i];
+      const float f = (float)counts[i];
+      hy[i] = f;
+      inex |= (double)f != counts[i];
+      if (cells[i] >= hf[s + 1]) {
+        tot[g * k + s] = t;
+        t = 0.0;
+        while (cells[i] >= hf[s + 1]) ++s;
+      }
+      t += counts[i];
     }
+    if (gene_ptr[g + 1] > gene_ptr[g]) tot[g * k + s] = t;
+    inexact[g] = inex;
   }, 16);
   gy32.upload(hy.data(), nnz, stream);
   // the f64 counts: widened on the device when every count is an f32 value
