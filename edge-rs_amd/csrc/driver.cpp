@@ -390,7 +390,11 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
         shape[g] = kIndptr;
         return;
       }
-      size_t np_ = 0;
+      // each subject's total in a register until the gene leaves that
+      // subject (cells increase, so each is visited once, in order)
+      size_t np_ = 0, run = 0;
+      double t = 0.0;
+      bool bad = false;
       for (uint32_t i = indptr[g]; i < indptr[g + 1]; ++i) {
         if (indices[i] >= n_cells) {
           shape[g] = kRange;
@@ -401,12 +405,20 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
           return;
         }
         const double y = data[i];
-        if (!(std::isfinite(y) && y >= 0.0)) value[g] = 1;
-        if (y > 0.0 && !value[g]) {
-          totals[g * n_runs + run_of[indices[i]]] += y;
+        if (!(std::isfinite(y) && y >= 0.0)) bad = true;
+        if (y > 0.0 && !bad) {
+          const size_t r = run_of[indices[i]];
+          if (r != run) {
+            totals[g * n_runs + run] = t;
+            t = 0.0;
+            run = r;
+          }
+          t += y;
           ++np_;
         }
       }
+      totals[g * n_runs + run] = t;
+      value[g] = bad ? 1 : 0;
       positives[g] = np_;
     }, 16);
     for (size_t g = 0; g < n_genes; ++g) {
