@@ -1,4 +1,5 @@
 // C ABI: request-level batch API, BOBYQA entry points and defaults.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -8,7 +9,9 @@
 #include "../../include/nebula_hip.h"
 #include "bobyqa.hpp"
 #include "driver.hpp"
+#include "persistent.hpp"
 #include "resident.hpp"
+#include "search_control.hpp"
 
 struct nh_resident {
   nh::Resident r;
@@ -72,6 +75,19 @@ int nh_resident_upload(size_t n_cells, size_t n_coef, size_t n_subjects, const d
       put_err(err, err_len, "empty subject block");
       return NH_ERR_SUBJECTS_NOT_CONTIGUOUS;
     }
+  if (gene_ptr[0] != 0) {
+    put_err(err, err_len, "gene_ptr must start at zero");
+    return NH_ERR_MALFORMED_SPARSE;
+  }
+  for (size_t g = 0; g < n_genes; ++g)
+    if (gene_ptr[g + 1] < gene_ptr[g]) {
+      put_err(err, err_len, "gene_ptr must be non-decreasing");
+      return NH_ERR_MALFORMED_SPARSE;
+    }
+  if (gene_ptr[n_genes] > 0 && (!cells || !counts)) {
+    put_err(err, err_len, "null cells or counts with stored entries");
+    return NH_ERR_INVALID_ARGUMENT;
+  }
   for (size_t g = 0; g < n_genes; ++g)
     for (uint64_t i = gene_ptr[g]; i < gene_ptr[g + 1]; ++i)
       if (cells[i] >= n_cells || (i > gene_ptr[g] && cells[i] <= cells[i - 1])) {
@@ -266,6 +282,83 @@ int nh_bobyqa_device_trace(int problem, int n, const double* x0, const double* l
   } catch (const std::exception& e) {
     put_err(err, err_len, e.what());
     return -100 - NH_ERR_GPU;
+  }
+}
+
+int nh_stage_two_search(int kind, const double* start, const double* lower, const double* upper,
+                        int ord, nh_search_fn f, void* ctx, double* x_out, int* found,
+                        int* failed, int* order_out, int* evals_out) {
+  if ((kind != nh::kSearchSubject && kind != nh::kSearchBoth) || (ord != 1 && ord != 3) ||
+      !start || !lower || !upper || !f || !x_out || !found || !failed || !order_out || !evals_out)
+    return NH_ERR_INVALID_ARGUMENT;
+  for (int i = 0; i < kind; ++i)
+    if (!(lower[i] < upper[i])) return NH_ERR_INVALID_BOUNDS;
+  const double lo[2] = {lower[0], kind == nh::kSearchBoth ? lower[1] : 0.0};
+  const double hi[2] = {upper[0], kind == nh::kSearchBoth ? upper[1] : 1.0};
+  const double st[2] = {start[0], start[1]};
+  nh::SearchControl c;
+  c.begin(kind, ord, st, nullptr, lo, hi, 0.0, 0.0);
+  while (c.active) {
+    double sub, cell;
+    c.point(sub, cell);
+    c.tell(f(sub, cell, c.point_order(), ctx));
+  }
+  double fv;
+  c.result(x_out, fv, *order_out);
+  *found = c.found;
+  *failed = c.failed;
+  *evals_out = c.evals;
+  return NH_OK;
+}
+
+int nh_stage_two_search_device(int problem, int kind, int ord, const double* start,
+                               const double* lower, const double* upper, int inj_from,
+                               int inj_to, int inj_order, int inj_kind, int32_t device,
+                               double* trace, int max_trace, double* x_out, int* found,
+                               int* failed, int* order_out, int* evals_out, char* err,
+                               size_t err_len) {
+  if (problem < 0 || problem > 4 || (kind != nh::kSearchSubject && kind != nh::kSearchBoth) ||
+      (ord != 1 && ord != 3) || inj_kind < 0 || inj_kind > 2 || max_trace < 0 || !start ||
+      !lower || !upper || (max_trace && !trace) || !x_out || !found || !failed || !order_out ||
+      !evals_out) {
+    put_err(err, err_len, "invalid arguments");
+    return NH_ERR_INVALID_ARGUMENT;
+  }
+  for (int i = 0; i < kind; ++i)
+    if (!(lower[i] < upper[i])) {
+      put_err(err, err_len, "invalid bounds");
+      return NH_ERR_INVALID_BOUNDS;
+    }
+  const double lo[2] = {lower[0], kind == nh::kSearchBoth ? lower[1] : 0.0};
+  const double hi[2] = {upper[0], kind == nh::kSearchBoth ? upper[1] : 1.0};
+  const double st2[2] = {start[0], start[1]};
+  try {
+    NH_HIP(hipSetDevice(device));
+    nh::DevBuf<double> dst, dlo, dhi, dtr;
+    nh::DevBuf<nh::DevSearchOut> dout;
+    hipStream_t st = nullptr;
+    dst.upload(st2, 2, st);
+    dlo.upload(lo, 2, st);
+    dhi.upload(hi, 2, st);
+    dtr.alloc((size_t)std::max(max_trace, 1) * 4);
+    dout.alloc(1);
+    NH_HIP(nh::launch_search_trace(problem, kind, ord, dst.p, dlo.p, dhi.p, inj_from, inj_to,
+                                   inj_order, inj_kind, dtr.p, max_trace, dout.p, st));
+    nh::DevSearchOut o;
+    NH_HIP(hipMemcpy(&o, dout.p, sizeof(o), hipMemcpyDeviceToHost));
+    const int rows = std::min(o.evals, max_trace);
+    if (rows > 0)
+      NH_HIP(hipMemcpy(trace, dtr.p, sizeof(double) * (size_t)rows * 4, hipMemcpyDeviceToHost));
+    x_out[0] = o.x[0];
+    x_out[1] = o.x[1];
+    *found = o.found;
+    *failed = o.failed;
+    *order_out = o.ord;
+    *evals_out = o.evals;
+    return NH_OK;
+  } catch (const std::exception& e) {
+    put_err(err, err_len, e.what());
+    return NH_ERR_GPU;
   }
 }
 

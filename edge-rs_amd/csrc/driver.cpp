@@ -23,6 +23,7 @@
 #include <numeric>
 #include <string>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/nebula_hip.h"
@@ -31,6 +32,7 @@
 #include "host_common.hpp"
 #include "kernels.hpp"
 #include "resident.hpp"
+#include "search_control.hpp"
 #include "stage_kernels.hpp"
 #include "stage_one.hpp"
 
@@ -368,6 +370,8 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
                                           std::to_string(n_coef));
   if (!indptr || !subject_id || !design) api_fail(NH_ERR_INVALID_ARGUMENT, "null input");
   if (indptr[0] != 0) api_fail(NH_ERR_MALFORMED_SPARSE, "indptr must start at zero");
+  if (indptr[n_genes] > 0 && (!indices || !data))
+    api_fail(NH_ERR_INVALID_ARGUMENT, "null indices or data with stored entries");
   // Cells' subject runs (checked for contiguity further down, after the
   // counts, as before), then one pass over the counts, genes in parallel:
   // structure and values checked, and for a valid gene its subject totals
@@ -438,15 +442,25 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
   P.params = p;
   P.fid.push_back(0);
   {
-    std::vector<uint64_t> seen{subject_id[0]};
+    // a subject seen in two runs: sort the runs' labels and look for a
+    // repeat (O(K log K)); the message names the first label, in cell
+    // order, whose run is not its first
+    std::vector<uint64_t> label{subject_id[0]};
     for (size_t i = 1; i < n_cells; ++i)
       if (subject_id[i] != subject_id[i - 1]) {
-        if (std::find(seen.begin(), seen.end(), subject_id[i]) != seen.end())
-          api_fail(NH_ERR_SUBJECTS_NOT_CONTIGUOUS, "subject " + std::to_string(subject_id[i]) + " is not one contiguous run");
-        seen.push_back(subject_id[i]);
+        label.push_back(subject_id[i]);
         P.fid.push_back(i);
       }
     P.fid.push_back(n_cells);
+    std::vector<uint64_t> sorted = label;
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) {
+      std::unordered_set<uint64_t> seen;
+      for (uint64_t l : label)
+        if (!seen.insert(l).second)
+          api_fail(NH_ERR_SUBJECTS_NOT_CONTIGUOUS,
+                   "subject " + std::to_string(l) + " is not one contiguous run");
+    }
   }
   P.k = P.fid.size() - 1;
   if (P.k != n_runs) api_fail(NH_ERR_INVALID_ARGUMENT, "internal: subject runs disagree");
@@ -802,9 +816,6 @@ void run_stage_one(Prepared& P, Resident& R, int host_threads, uint64_t& rounds)
         pl.refit = weak ? 2 : 0;
       }
     }
-    if (pl.refit == 2)
-      api_fail(NH_ERR_INVALID_ARGUMENT,
-               "NEBULA-LN's one-component refit is not implemented on this path; use method HL");
   }
 }
 
@@ -871,54 +882,38 @@ void compute_features(Prepared& P) {
 }
 
 // ---------------------------------------------------------------------------
-// Stage two, plain engine: one BOBYQA per gene on the host, all live genes'
-// points in one fit launch per round (edge-rs search_all, stage_two.rs:362-544,
-// without the cohorts).  Inner fits start from the gene's first finite fit
-// (stage_two.rs:36-43).
+// Stage two, plain engine: one search per gene on the host (SearchControl:
+// BOBYQA over both components, or the subject variance alone for NEBULA-LN's
+// weak genes), all live searches' points in one fit launch per round (edge-rs
+// search_all, stage_two.rs:362-544, without the cohorts); a one-component
+// search's polish stencil goes out as its three points in one round.  Inner
+// fits start from the gene's first finite fit (stage_two.rs:36-43).
 void run_stage_two_plain(Prepared& P, Resident& R, int waves, nebula_run_stats_t& st) {
   const size_t G = P.kept.size(), nb = P.n_coef, k = P.k;
   const nebula_params_t& p = P.params;
   struct Search {
-    Bobyqa<2> b;
-    unsigned order;
-    bool active = false, found = false, failed = false;
+    SearchControl c;
     std::vector<double> warm_beta, warm_lw;
-    int evals = 0;
   };
   std::vector<Search> S(G);
   const double lo[2] = {p.min_subject, p.min_cell}, hi[2] = {p.max_subject, p.max_cell};
-  auto begin_pass = [&](size_t i) {
-    Search& s = S[i];
-    const GenePlan& pl = P.plans[i];
-    double x0[2] = {std::min(std::max(pl.sigma, lo[0]), hi[0]),
-                    std::min(std::max(pl.gamma, lo[1]), hi[1])};
-    s.b.init(2, x0, lo, hi);
-    s.active = s.b.status != BOBYQA_BAD_BOX;
-  };
-  auto end_pass = [&](size_t i, bool found) {
-    Search& s = S[i];
-    if (found || s.order == 1) {
-      s.active = false;
-      s.found = found;
-      if (found) {
-        P.plans[i].found = true;
-        P.plans[i].v[0] = s.b.xbest_out[0];
-        P.plans[i].v[1] = s.b.xbest_out[1];
-        P.plans[i].objective = s.b.fbest_out;
-        P.plans[i].failed = s.b.status == BOBYQA_ROUNDOFF_LIMITED;
-      }
-      return;
-    }
-    s.order = 1;
-    begin_pass(i);
-    if (!s.active) s.found = false;
+  auto finish = [&](size_t i) {
+    GenePlan& pl = P.plans[i];
+    const SearchControl& c = S[i].c;
+    pl.found = c.found != 0;
+    pl.failed = c.failed != 0;
+    int ord;
+    c.result(pl.v, pl.objective, ord);
+    pl.evaluations = c.evals;
   };
   for (size_t i = 0; i < G; ++i) {
-    if (P.plans[i].refit != 1) continue;
-    S[i].order = P.plans[i].ord;
-    P.plans[i].searched = true;
-    begin_pass(i);
-    if (!S[i].active) end_pass(i, false);
+    GenePlan& pl = P.plans[i];
+    if (pl.refit == 0) continue;
+    const double start[2] = {pl.sigma, pl.gamma};
+    S[i].c.begin(pl.refit == 1 ? kSearchBoth : kSearchSubject, (int)pl.ord, start, nullptr, lo,
+                 hi, 0.0, 0.0);
+    pl.searched = true;
+    if (!S[i].c.active) finish(i);
     ++st.genes_searched;
   }
   std::vector<DevRequest> req;
@@ -932,20 +927,30 @@ void run_stage_two_plain(Prepared& P, Resident& R, int waves, nebula_run_stats_t
     lwi.clear();
     who.clear();
     for (size_t i = 0; i < G; ++i) {
-      Search& s = S[i];
-      if (!s.active) continue;
-      const double subject = s.b.xask[0], cell = s.b.xask[1];
-      req.push_back(DevRequest{(uint32_t)i, s.order, subject, cell});
-      if (!s.warm_beta.empty()) {
-        bi.insert(bi.end(), s.warm_beta.begin(), s.warm_beta.end());
-        lwi.insert(lwi.end(), s.warm_lw.begin(), s.warm_lw.end());
-      } else {
-        std::vector<double> b0 = P.plans[i].beta_start;
-        b0[P.intercept] -= subject / 2.0;
-        bi.insert(bi.end(), b0.begin(), b0.end());
-        lwi.insert(lwi.end(), k, 0.0);
+      SearchControl& c = S[i].c;
+      if (!c.active) continue;
+      // every point the search waits for: one, or a polish stencil's three
+      const int n_pts = c.kind == kSearchSubject ? c.ss.n_ask : 1;
+      for (int q = 0; q < n_pts; ++q) {
+        double subject, cell;
+        if (c.kind == kSearchSubject) {
+          subject = c.ss.ask[q];
+          cell = c.cold[1];
+        } else {
+          c.point(subject, cell);
+        }
+        req.push_back(DevRequest{(uint32_t)i, (uint32_t)c.point_order(), subject, cell});
+        if (!S[i].warm_beta.empty()) {
+          bi.insert(bi.end(), S[i].warm_beta.begin(), S[i].warm_beta.end());
+          lwi.insert(lwi.end(), S[i].warm_lw.begin(), S[i].warm_lw.end());
+        } else {
+          std::vector<double> b0 = P.plans[i].beta_start;
+          b0[P.intercept] -= subject / 2.0;
+          bi.insert(bi.end(), b0.begin(), b0.end());
+          lwi.insert(lwi.end(), k, 0.0);
+        }
+        who.push_back(i);
       }
-      who.push_back(i);
     }
     if (who.empty()) break;
     R.solve(req, bi, &lwi, p.eps, 0, waves, out, ob, olw, nullptr, true);
@@ -961,21 +966,17 @@ void run_stage_two_plain(Prepared& P, Resident& R, int waves, nebula_run_stats_t
       if (fin)
         v = profile_objective(req[q].subject, req[q].cell, r.ll, r.ll_prev, r.log_det, r.second,
                               (double)P.n_cells, (double)k, P.genes[i].tail);
-      ++s.evals;
       ++st.evaluations;
-      if (std::isfinite(v) && s.warm_beta.empty()) {
+      // a stencil's later points were asked with the start this round began
+      // with; the first finite fit of the gene becomes the start after it
+      const bool was_cold = s.warm_beta.empty();
+      if (s.c.tell(v) && was_cold) {
         s.warm_beta.assign(ob.begin() + q * nb, ob.begin() + (q + 1) * nb);
         s.warm_lw.assign(olw.begin() + q * k, olw.begin() + (q + 1) * k);
       }
-      if (!std::isfinite(v)) {
-        end_pass(i, false);
-        continue;
-      }
-      s.b.tell(v);
-      if (!s.b.want) end_pass(i, true);
+      if (!s.c.active) finish(i);
     }
   }
-  for (size_t i = 0; i < G; ++i) P.plans[i].evaluations = S[i].evals;
 }
 
 // ---------------------------------------------------------------------------

@@ -14,9 +14,11 @@
 // and tells the stepper; a finished workgroup takes the next gene.  No host
 // round trip until every gene is done.
 //
-// Search semantics are the host plain engine's: the order-one retry and the
-// abandon-on-non-finite rule of StageTwoSearch (nebula.rs:1656-1741), inner
-// fits warm-started from the gene's first finite fit (stage_two.rs:36-43).
+// Search semantics are the host plain engine's (SearchControl, shared with
+// it): BOBYQA over both components or NEBULA-LN's one-component simplex
+// search, the order-one retry and the abandon-on-non-finite rules of
+// StageTwoSearch (nebula.rs:1568-1951), inner fits warm-started from the
+// gene's first finite fit (stage_two.rs:36-43).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,6 +31,7 @@
 
 #include "bobyqa.hpp"
 #include "driver.hpp"
+#include "search_control.hpp"
 #include "pml_fit.hpp"
 #include "persistent.hpp"
 #include "mlp.hpp"
@@ -53,33 +56,36 @@ __device__ static double device_objective(double subject, double cell, const Fit
   return isfinite(v) ? -v : INFINITY;
 }
 
-// The stepper runs on one lane between fits; kept out of line so the sweep
-// loop's register allocation is not shaped by it.
-__device__ __noinline__ static void bq_init(Bobyqa<2>* b, const double* x0, const double* lo,
-                                            const double* hi) {
-  b->init(2, x0, lo, hi);
+// The search control runs on one lane between fits; kept out of line so the
+// sweep loop's register allocation is not shaped by it.
+__device__ __noinline__ static void ctl_begin(SearchControl* c, const DevSearch* S,
+                                              const float* pred, const PersistentArgs* A) {
+  c->begin((int)S->kind, (int)S->ord, S->start, S->kind == kSearchBoth ? pred : nullptr, A->lo,
+           A->hi, A->warm_step, A->warm_xtol);
 }
-__device__ __noinline__ static void bq_init_step(Bobyqa<2>* b, const double* x0, const double* lo,
-                                                 const double* hi, const double* step,
-                                                 double xtol) {
-  b->init_with_step(2, x0, lo, hi, step, xtol, 1000);
+__device__ __noinline__ static int ctl_tell(SearchControl* c, double v) { return c->tell(v); }
+__device__ __noinline__ static void ctl_out(const SearchControl* c, DevSearchOut* o) {
+  double f;
+  int ord;
+  c->result(o->x, f, ord);
+  o->f = f;
+  o->found = c->found;
+  o->failed = c->failed;
+  o->evals = c->evals;
+  o->ord = ord;
+  o->warm_rejected = c->rejected;
 }
-__device__ __noinline__ static void bq_tell(Bobyqa<2>* b, double v) { b->tell(v); }
-
-enum SearchPhase { PH_COLD = 0, PH_WARM = 1, PH_CHECK = 2 };
 
 template <int NB, int WAVES>
 __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<NB>)))
 void stage_two_kernel(PersistentArgs A) {
   constexpr int NT = WAVES * 64;
   __shared__ FitLds<NB, WAVES> sh;
-  __shared__ Bobyqa<2> bq;
+  __shared__ SearchControl ctl;
   __shared__ double warm_beta[NB];
   __shared__ double red[NT / 64];
-  __shared__ int s_slot, s_active, s_found, s_failed, s_have_warm, s_evals, s_order, s_ok;
-  __shared__ int s_phase, s_wfailed, s_rejected, s_from_warm;
-  __shared__ double s_subject, s_cell, s_value, s_wx[2], s_wf, s_cold[2], s_warm[2];
-  __shared__ double s_anchor_f;  // objective at the inner fits' current start
+  __shared__ int s_slot, s_ok;
+  __shared__ double s_subject, s_cell;
   extern __shared__ double nh_s2_dyn[];
   const int tid = threadIdx.x;
   const DeviceData& dd = A.dd;
@@ -88,31 +94,6 @@ void stage_two_kernel(PersistentArgs A) {
   double* base = A.work_in_lds ? nh_s2_dyn : A.scratch + (size_t)blockIdx.x * A.stride;
   wk.carve(base, k, WAVES);
   double* warm_lw = base + FitWork<NB>::doubles(k, WAVES);
-  const double lo[2] = {A.lo[0], A.lo[1]}, hi[2] = {A.hi[0], A.hi[1]};
-  // (thread 0) starts a BOBYQA pass of the given phase at the current order
-  auto begin_search = [&](int phase) {
-    s_phase = phase;
-    if (phase == PH_WARM) {
-      // The initial step is a fraction of the predicted value, widened to
-      // twice the distance to stage one's estimate where the two disagree
-      // (a prediction far from stage one is less certain), at most to a
-      // tenth of the predicted value: wider first steps cost more BOBYQA
-      // stages than they save where the network is right and stage one is
-      // not.  The final precision rhoend = warm_xtol * warm_step * x stays
-      // the same whatever the step: xtol is scaled down when it is widened.
-      double step[2];
-      for (int i = 0; i < 2; ++i) {
-        const double base = A.warm_step * s_warm[i];
-        step[i] = fmax(base, fmin(2.0 * fabs(s_warm[i] - s_cold[i]), 0.1 * s_warm[i]));
-        step[i] = fmin(fmax(step[i], 1e-3 * lo[i]), 0.45 * (hi[i] - lo[i]));
-      }
-      const double xtol = A.warm_xtol * fmin(1.0, A.warm_step * s_warm[0] / step[0]);
-      bq_init_step(&bq, s_warm, lo, hi, step, xtol);
-    } else {
-      bq_init(&bq, s_cold, lo, hi);
-    }
-    s_active = bq.status != BOBYQA_BAD_BOX && bq.want;
-  };
   for (;;) {
     if (tid == 0) s_slot = (int)atomicAdd(A.queue, 1u);
     __syncthreads();
@@ -120,46 +101,27 @@ void stage_two_kernel(PersistentArgs A) {
     __syncthreads();
     if (slot >= A.n_search) break;
     const DevSearch S = A.search[A.order[slot]];
-    if (tid == 0) {
-      s_order = (int)S.ord;
-      s_have_warm = 0;
-      s_evals = 0;
-      s_found = 0;
-      s_failed = 0;
-      s_rejected = 0;
-      s_from_warm = 0;
-      s_cold[0] = fmin(fmax(S.start[0], A.lo[0]), A.hi[0]);
-      s_cold[1] = fmin(fmax(S.start[1], A.lo[1]), A.hi[1]);
-      s_active = 0;
-      if (A.pred) {
-        const float* pr = A.pred + 2 * (size_t)A.order[slot];
-        s_warm[0] = fmin(fmax(exp((double)pr[0]), A.lo[0]), A.hi[0]);
-        s_warm[1] = fmin(fmax(exp(-(double)pr[1]), A.lo[1]), A.hi[1]);
-        begin_search(PH_WARM);
-      }
-      if (!s_active) begin_search(PH_COLD);
-    }
+    if (tid == 0)
+      ctl_begin(&ctl, &A.search[A.order[slot]],
+                A.pred ? A.pred + 2 * (size_t)A.order[slot] : nullptr, &A);
     __syncthreads();
-    while (s_active) {
-      // request for the asked point (the cold start when checking a warm result)
-      if (tid == 0) {
-        s_subject = s_phase == PH_CHECK ? s_cold[0] : bq.xask[0];
-        s_cell = s_phase == PH_CHECK ? s_cold[1] : bq.xask[1];
-      }
+    while (ctl.active) {
+      if (tid == 0) ctl.point(s_subject, s_cell);
       __syncthreads();
       const double subject = s_subject, cell = s_cell;
+      const int have_warm = ctl.have_warm;
       if (tid < NB) {
         double b = S.beta_start[tid];
         if (tid == (int)A.intercept) b -= subject / 2.0;
-        sh.beta[tid] = s_have_warm ? warm_beta[tid] : b;
+        sh.beta[tid] = have_warm ? warm_beta[tid] : b;
       }
-      for (int s = tid; s < k; s += NT) wk.lw[s] = s_have_warm ? warm_lw[s] : 0.0;
+      for (int s = tid; s < k; s += NT) wk.lw[s] = have_warm ? warm_lw[s] : 0.0;
       __syncthreads();
       const double exps = exp(subject);
       const double alpha = 1.0 / (exps - 1.0);
       const double lambda = 1.0 / (sqrt(exps) * (exps - 1.0));
       Fitter<NB, WAVES> f(dd, sh, wk, A.counters, S.gene, cell, alpha, lambda);
-      const FitResult r = f.run(0, (unsigned)s_order, A.knobs, s_evals > 0);
+      const FitResult r = f.run(0, (unsigned)ctl.point_order(), A.knobs, ctl.evals > 0);
       // finite point and the lgamma tail over the count histogram
       double part = 0.0;
       bool fin = true;
@@ -182,13 +144,7 @@ void stage_two_kernel(PersistentArgs A) {
         if (allfin)
           v = device_objective(subject, cell, r, (double)dd.n_cells, (double)k, tail, S.n_pos,
                                S.n_one, S.n_two);
-        s_value = v;
-        s_evals += 1;
-        // Inner fits start from the gene's first finite fit (the cold search,
-        // as edge-rs does); the warm search moves that start to its best
-        // point so far, which its steps stay close to.
-        s_ok = isfinite(v) && (!s_have_warm || (s_phase == PH_WARM && v < s_anchor_f));
-        if (s_ok) s_anchor_f = v;
+        s_ok = ctl_tell(&ctl, v);
       }
       __syncthreads();
       if (s_ok) {
@@ -196,79 +152,101 @@ void stage_two_kernel(PersistentArgs A) {
         for (int s = tid; s < k; s += NT) warm_lw[s] = wk.lw[s];
       }
       __syncthreads();
-      if (tid == 0) {
-        if (s_ok) s_have_warm = 1;
-        const double v = s_value;
-        if (s_phase == PH_CHECK) {
-          // keep the warm result unless the cold start point does better
-          if (isfinite(v) && s_wf <= v) {
-            s_active = 0;
-            s_found = 1;
-            s_failed = s_wfailed;
-            s_from_warm = 1;
-          } else {
-            s_rejected = 1;
-            s_order = (int)S.ord;
-            s_have_warm = 0;
-            begin_search(PH_COLD);
-          }
-        } else {
-          bool pass_over = false, found = false;
-          if (!isfinite(v)) {
-            pass_over = true;
-          } else {
-            bq_tell(&bq, v);
-            if (!bq.want) {
-              pass_over = true;
-              found = true;
-            }
-          }
-          if (pass_over) {
-            if (found || s_order == 1) {
-              if (s_phase == PH_WARM && found) {
-                s_wx[0] = bq.xbest_out[0];
-                s_wx[1] = bq.xbest_out[1];
-                s_wf = bq.fbest_out;
-                s_wfailed = bq.status == BOBYQA_ROUNDOFF_LIMITED ? 1 : 0;
-                s_phase = PH_CHECK;
-              } else if (s_phase == PH_WARM) {
-                s_rejected = 1;
-                s_order = (int)S.ord;
-                s_have_warm = 0;
-                begin_search(PH_COLD);
-              } else {
-                s_active = 0;
-                s_found = found ? 1 : 0;
-                s_failed = bq.status == BOBYQA_ROUNDOFF_LIMITED ? 1 : 0;
-              }
-            } else {
-              s_order = 1;
-              begin_search(s_phase);
-              if (!s_active && s_phase == PH_WARM) {
-                s_rejected = 1;
-                s_order = (int)S.ord;
-                begin_search(PH_COLD);
-              }
-            }
-          }
-        }
-      }
-      __syncthreads();
     }
-    if (tid == 0) {
-      DevSearchOut o;
-      o.x[0] = s_from_warm ? s_wx[0] : bq.xbest_out[0];
-      o.x[1] = s_from_warm ? s_wx[1] : bq.xbest_out[1];
-      o.f = s_from_warm ? s_wf : bq.fbest_out;
-      o.found = s_found;
-      o.failed = s_failed;
-      o.evals = s_evals;
-      o.ord = s_order;
-      o.warm_rejected = s_rejected;
-      A.out[A.order[slot]] = o;
-    }
+    if (tid == 0) ctl_out(&ctl, &A.out[A.order[slot]]);
     __syncthreads();
   }
+}
+
+// ---------------------------------------------------------------------------
+// The persistent kernel's search control (the same ctl_begin / ctl_tell /
+// ctl_out) driven on the device by a built-in objective instead of a fit, so
+// the state machine's branches can be checked against the reference's cases.
+// Objectives of (subject s, cell c): 0 (s-0.3)^2 + 0.01 (c-4)^2; 1 the valley
+// (s+c-1)^2; 2 s; 3 (s-0.3)^2 + 2; 4 a shallow well at the lower bound and a
+// deeper one at 0.05.  Injection: calls inj_from..inj_to (one-based) at an
+// order >= inj_order return +inf (inj_kind 0), NaN (1) or the profile
+// objective of a fit whose Laplace correction fell below -1 (2).
+__device__ static double search_test_objective(int problem, double s, double c) {
+  NH_STRICT_FP
+  switch (problem) {
+    case 0: {
+      const double a = s - 0.3, b = c - 4.0;
+      return a * a + 0.01 * (b * b);
+    }
+    case 1: {
+      const double a = s + c - 1.0;
+      return a * a;
+    }
+    case 2:
+      return s;
+    case 3: {
+      const double a = s - 0.3;
+      return a * a + 2.0;
+    }
+    default: {
+      const double a = (s - 1e-4) / 2e-3, b = (s - 0.05) / 0.02;
+      return -0.5 * exp(-(a * a)) - 1.0 * exp(-(b * b));
+    }
+  }
+}
+
+__global__ void search_trace_kernel(int problem, int kind, int ord, const double* start,
+                                    const double* lo, const double* hi, int inj_from, int inj_to,
+                                    int inj_order, int inj_kind, double* trace, int max_trace,
+                                    DevSearchOut* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  __shared__ SearchControl ctl;
+  DevSearch S{};
+  S.kind = (uint32_t)kind;
+  S.ord = (uint32_t)ord;
+  S.start[0] = start[0];
+  S.start[1] = start[1];
+  PersistentArgs A{};
+  A.lo[0] = lo[0];
+  A.lo[1] = lo[1];
+  A.hi[0] = hi[0];
+  A.hi[1] = hi[1];
+  ctl_begin(&ctl, &S, nullptr, &A);
+  int call = 0;
+  while (ctl.active) {
+    double sub, cell;
+    ctl.point(sub, cell);
+    const int order = ctl.point_order();
+    ++call;
+    double v = search_test_objective(problem, sub, cell);
+    if (call >= inj_from && call <= inj_to && order >= inj_order) {
+      if (inj_kind == 0) {
+        v = INFINITY;
+      } else if (inj_kind == 1) {
+        v = NAN;
+      } else {
+        FitResult r{};
+        r.ll = -v;
+        r.ll_prev = -v;
+        r.second = -1.5;
+        v = device_objective(sub, cell, r, 100.0, 4.0, 0.0, 10.0, 3.0, 2.0);
+      }
+    }
+    if (call <= max_trace) {
+      double* row = trace + (size_t)(call - 1) * 4;
+      row[0] = sub;
+      row[1] = cell;
+      row[2] = (double)order;
+      row[3] = v;
+    }
+    ctl_tell(&ctl, v);
+  }
+  ctl_out(&ctl, out);
+}
+
+hipError_t launch_search_trace(int problem, int kind, int ord, const double* start,
+                               const double* lo, const double* hi, int inj_from, int inj_to,
+                               int inj_order, int inj_kind, double* trace, int max_trace,
+                               DevSearchOut* out, hipStream_t st) {
+  hipLaunchKernelGGL(search_trace_kernel, dim3(1), dim3(64), 0, st, problem, kind, ord, start, lo,
+                     hi, inj_from, inj_to, inj_order, inj_kind, trace, max_trace, out);
+  return hipGetLastError();
 }
 
 __global__ void reset_queue_kernel(unsigned* q) {
@@ -350,12 +328,13 @@ static std::shared_ptr<StageTwoPlan> build_stage_two_plan(const Prepared& P, Res
   std::vector<DevSearch> hs;
   std::vector<double> tv, tm;
   for (size_t i = 0; i < G; ++i) {
-    if (P.plans[i].refit != 1) continue;
+    if (P.plans[i].refit == 0) continue;
     const GenePlan& pl = P.plans[i];
     const GeneInfo& gi = P.genes[i];
     DevSearch d{};
     d.gene = (uint32_t)i;
     d.ord = pl.ord;
+    d.kind = pl.refit == 1 ? kSearchBoth : kSearchSubject;
     d.start[0] = pl.sigma;
     d.start[1] = pl.gamma;
     for (size_t j = 0; j < nb; ++j) d.beta_start[j] = pl.beta_start[j];
@@ -441,7 +420,7 @@ void run_stage_two_batched(Prepared& P, Resident& R, const nebula_hip_options_t&
   const size_t G = P.kept.size(), nb = P.n_coef;
   size_t n = 0;
   for (size_t i = 0; i < G; ++i)
-    if (P.plans[i].refit == 1) {
+    if (P.plans[i].refit != 0) {
       P.plans[i].searched = true;
       ++n;
     }

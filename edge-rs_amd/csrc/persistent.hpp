@@ -12,7 +12,8 @@ namespace nh {
 struct DevSearch {
   uint32_t gene;
   uint32_t ord;          // Laplace order of the first pass
-  double start[2];       // stage one's (sigma, gamma)
+  uint32_t kind;         // SearchKind: 2 both components, 1 the subject variance alone
+  double start[2];       // stage one's (sigma, gamma); gamma is held for kind 1
   double beta_start[8];  // fixed effects every cold fit starts from
   uint64_t tail_off;     // count histogram slice (values != 1, 2)
   uint32_t tail_len;
@@ -48,6 +49,10 @@ struct PersistentArgs {
 };
 
 size_t persistent_stride(int nb, int k, int waves);
+hipError_t launch_search_trace(int problem, int kind, int ord, const double* start,
+                               const double* lo, const double* hi, int inj_from, int inj_to,
+                               int inj_order, int inj_kind, double* trace, int max_trace,
+                               DevSearchOut* out, hipStream_t st);
 hipError_t launch_persistent(const PersistentArgs& A, int waves, int grid, hipStream_t st);
 
 }  // namespace nh

@@ -457,6 +457,80 @@ def bobyqa_device_trace(problem, x0, lower, upper, max_trace=2000):
     return tr[: rows * (n + 1)].reshape(rows, n + 1), status
 
 
+SEARCH_FN = C.CFUNCTYPE(C.c_double, C.c_double, C.c_double, C.c_int, C.c_void_p)
+SEARCH_BOTH = 2
+SEARCH_SUBJECT = 1
+
+
+def stage_two_search(f, kind, start, lower, upper, ord=1):
+    """The stage-two search control both engines run (nh_stage_two_search),
+    on the host against f(subject, cell, order).  kind 2: BOBYQA over both
+    components; kind 1: the subject variance alone, cell held at start[1].
+    Returns dict(x, found, failed, order, evaluations)."""
+    st = np.zeros(2)
+    st[: len(start)] = start
+    lo = np.zeros(2)
+    lo[: len(lower)] = lower
+    hi = np.ones(2)
+    hi[: len(upper)] = upper
+
+    @SEARCH_FN
+    def cb(sub, cell, order, _ctx):
+        return float(f(sub, cell, order))
+
+    x = np.zeros(2)
+    found, failed, order, ev = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    L = lib()
+    L.nh_stage_two_search.restype = C.c_int
+    L.nh_stage_two_search.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, SEARCH_FN, C.c_void_p,
+                                      _dp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    rc = L.nh_stage_two_search(kind, st.ctypes.data_as(_dp), lo.ctypes.data_as(_dp),
+                               hi.ctypes.data_as(_dp), ord, cb, None, x.ctypes.data_as(_dp),
+                               C.byref(found), C.byref(failed), C.byref(order), C.byref(ev))
+    if rc != 0:
+        _raise(rc, C.create_string_buffer(b"nh_stage_two_search rejected its arguments"))
+    return dict(x=x, found=bool(found.value), failed=bool(failed.value), order=order.value,
+                evaluations=ev.value)
+
+
+def stage_two_search_device(problem, kind, start, lower, upper, ord=1, inject=None,
+                            max_trace=4000, device=0):
+    """The same control on the device through the persistent kernel's entry
+    points, against built-in objective `problem` (nh_stage_two_search_device).
+    inject = (from_call, to_call, min_order, kind) with kind 0 +inf, 1 NaN,
+    2 a Laplace correction below -1.  Returns the dict of stage_two_search
+    plus trace rows (subject, cell, order, value)."""
+    st = np.zeros(2)
+    st[: len(start)] = start
+    lo = np.zeros(2)
+    lo[: len(lower)] = lower
+    hi = np.ones(2)
+    hi[: len(upper)] = upper
+    inj = inject or (0, -1, 99, 0)
+    tr = np.zeros(max_trace * 4)
+    x = np.zeros(2)
+    found, failed, order, ev = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    err = C.create_string_buffer(512)
+    L = lib()
+    L.nh_stage_two_search_device.restype = C.c_int
+    L.nh_stage_two_search_device.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int,
+                                             C.c_int, C.c_int, C.c_int, C.c_int32, _dp, C.c_int,
+                                             _dp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                             C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                             C.c_char_p, C.c_size_t]
+    rc = L.nh_stage_two_search_device(problem, kind, ord, st.ctypes.data_as(_dp),
+                                      lo.ctypes.data_as(_dp), hi.ctypes.data_as(_dp), *inj,
+                                      device, tr.ctypes.data_as(_dp), max_trace,
+                                      x.ctypes.data_as(_dp), C.byref(found), C.byref(failed),
+                                      C.byref(order), C.byref(ev), err, 512)
+    if rc != 0:
+        _raise(rc, err)
+    rows = min(ev.value, max_trace)
+    return dict(x=x, found=bool(found.value), failed=bool(failed.value), order=order.value,
+                evaluations=ev.value, trace=tr[: rows * 4].reshape(rows, 4))
+
+
 def glm_sc_test(coefficients, covariance, coef=None, contrast=None):
     """Wald test of one coefficient (coef) or a contrast (n_coef weights) per
     gene on a fit's coefficients (genes x n_coef) and packed covariance

@@ -312,6 +312,41 @@ int nh_bobyqa_device_trace(int problem, int n, const double* x0,
                            double* trace, int max_trace, int* evals_out,
                            char* err, size_t err_len);
 
+/* One gene's stage-two search control, the state machine both engines run
+ * (StageTwoSearch, src/sc/nebula.rs:1568-1951), on the host against a caller's
+ * objective f(subject, cell, order): kind 2 searches both variance components
+ * with BOBYQA from start[0..2); kind 1 the subject variance alone from
+ * start[0], cell held at start[1] (NEBULA-LN's one-component refit: bounded
+ * Nelder-Mead, restart off a bound, check, three quadratic polish stencils).
+ * ord 1 or 3 is the Laplace order of the first pass; a pass at order 3 that
+ * meets a non-finite value is repeated at order 1.  Writes the minimiser
+ * (x_out[0..2), the held cell for kind 1), whether one was found, whether
+ * BOBYQA ended RoundoffLimited (nebula's -50), the final order and the
+ * evaluation count.  Returns NH_OK, NH_ERR_INVALID_ARGUMENT or
+ * NH_ERR_INVALID_BOUNDS (lower[i] >= upper[i]). */
+typedef double (*nh_search_fn)(double subject, double cell, int order, void* ctx);
+int nh_stage_two_search(int kind, const double* start, const double* lower,
+                        const double* upper, int ord, nh_search_fn f, void* ctx,
+                        double* x_out, int* found, int* failed, int* order_out,
+                        int* evals_out);
+
+/* The same control run on lane 0 of a device workgroup through the
+ * persistent kernel's entry points, against a built-in objective of (s, c):
+ * problem 0 (s-0.3)^2 + 0.01 (c-4)^2, 1 (s+c-1)^2, 2 s, 3 (s-0.3)^2 + 2,
+ * 4 a shallow well at the lower bound and a deeper one at s = 0.05.  Calls
+ * inj_from..inj_to (one-based) evaluated at order >= inj_order return +inf
+ * (inj_kind 0), NaN (1) or the profile objective of a fit whose Laplace
+ * correction is below -1 (2, +inf by VarianceObjective::assemble,
+ * src/sc/nebula.rs:1452-1456).  trace: max_trace rows of (subject, cell,
+ * order, value).  No counterpart in edge-rs (its state machine is tested on
+ * the host only).  Returns NH_OK or an NH_ERR_* code. */
+int nh_stage_two_search_device(int problem, int kind, int ord, const double* start,
+                               const double* lower, const double* upper, int inj_from,
+                               int inj_to, int inj_order, int inj_kind, int32_t device,
+                               double* trace, int max_trace, double* x_out, int* found,
+                               int* failed, int* order_out, int* evals_out, char* err,
+                               size_t err_len);
+
 /* The FP64 sweep's exp (which 0), log (1) and reciprocal (2), and the device
  * objective's log Gamma for x > 0 (3), applied on the device to x[0..n), for
  * numerics tests of the device fit (no counterpart in edge-rs, whose fits call

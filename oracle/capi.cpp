@@ -126,6 +126,29 @@ int oracle_profile_objective(std::size_t n_genes, std::size_t n_cells,
   }
 }
 
+// StageTwoSearch on a caller's objective f(x, n, order, call, ctx), the
+// reference's run_search harness (src/sc/nebula.rs:3516-3542): n = 2 runs
+// BOBYQA, n = 1 the simplex search.  Returns 1 when a minimiser was found
+// (written to x_out), 0 when the search was abandoned.
+typedef double (*oracle_search_fn)(const double* x, int n, int order, int call, void* ctx);
+int oracle_stage_two_search(int n, const double* start, const double* lo, const double* hi,
+                            int ord, oracle_search_fn f, void* ctx, double* x_out,
+                            int* failed, int* order_out, int* evaluations) {
+  int calls = 0;
+  std::vector<double> s(start, start + n), l(lo, lo + n), h(hi, hi + n);
+  SearchOutcome r = stage_two_search(s, l, h, (unsigned)ord,
+                                     [&](const std::vector<double>& x, unsigned order) {
+                                       ++calls;
+                                       return f(x.data(), n, (int)order, calls, ctx);
+                                     });
+  if (r.found)
+    for (int j = 0; j < n; ++j) x_out[j] = r.x[j];
+  *failed = r.failed ? 1 : 0;
+  *order_out = (int)r.order;
+  *evaluations = r.evaluations;
+  return r.found ? 1 : 0;
+}
+
 void oracle_fit_free(nebula_fit_t* f) {
   std::free(f->coefficients);
   std::free(f->covariance);

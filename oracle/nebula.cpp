@@ -17,12 +17,17 @@
 // their side effect is kept, the choice of the quasi-Newton for stage one
 // when they would apply (zeros.rs:219-262's 32-cell / 50% rule).
 // NEBULA-LN's one-component refit (Nelder-Mead plus polish, nebula.rs:1798-1950)
-// is not restated; a gene that would take it is reported as an error.
+// is restated in search.hpp.
+//
+// Test infrastructure: only the tests, the smoke check and the benchmark's CPU
+// baseline load this library, as the checker.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <map>
 #include <mutex>
@@ -32,6 +37,7 @@
 #include "../edge-rs_amd/csrc/bobyqa.hpp"
 #include "lbfgsb.hpp"
 #include "oracle.hpp"
+#include "search.hpp"
 
 namespace oracle {
 
@@ -360,6 +366,9 @@ struct Objective {
   double n_cells, k;
   const std::vector<double>* counts;
   double n_pos, n_one, n_two;
+  // NEBULA-LN's one-component restriction: the cell-level size held fixed
+  // (VarianceObjective::fixed_cell, nebula.rs:1388); NaN for both components
+  double fixed_cell = std::numeric_limits<double>::quiet_NaN();
   double tail(double cell) const {
     double t = 0.0;
     for (double y : *counts)
@@ -378,7 +387,7 @@ struct Objective {
     return std::isfinite(ll) ? -ll : std::numeric_limits<double>::infinity();
   }
   double value(const double* x) const {
-    double subject = x[0], cell = x[1];
+    double subject = x[0], cell = std::isnan(fixed_cell) ? x[1] : fixed_cell;
     if (!(std::isfinite(subject) && subject > 0.0 && std::isfinite(cell) && cell > 0.0))
       return std::numeric_limits<double>::infinity();
     std::vector<double> beta = *beta_start;
@@ -394,22 +403,22 @@ struct Objective {
 
 // StageTwoSearch over both components (nebula.rs:1743-1796): one BOBYQA pass,
 // retried from the start at order one when a raised-order pass hit a
-// rejected evaluation.  Returns found, sets x/failed.
-bool refine_variance(Objective obj, unsigned ord, const double start[2],
-                     const double lo[2], const double hi[2], double x[2],
-                     bool& failed, StageTwoInfo* info) {
+// rejected evaluation.  f(x, order) is the objective.
+template <class F>
+SearchOutcome bobyqa_search(F&& f, unsigned ord, const double start[2], const double lo[2],
+                            const double hi[2], double* fbest) {
+  SearchOutcome out;
   unsigned order = ord;
-  int evals = 0;
   for (;;) {
-    obj.params.ord = order;
     double x0[2] = {std::min(std::max(start[0], lo[0]), hi[0]),
                     std::min(std::max(start[1], lo[1]), hi[1])};
     nh::Bobyqa<2> b;
     b.init(2, x0, lo, hi);
     bool found = b.status != nh::BOBYQA_BAD_BOX;
     while (found && b.want) {
-      double v = obj.value(b.xask);
-      ++evals;
+      std::vector<double> xv{b.xask[0], b.xask[1]};
+      double v = f(xv, order);
+      ++out.evaluations;
       if (!std::isfinite(v)) {
         found = false;
         break;
@@ -417,20 +426,50 @@ bool refine_variance(Objective obj, unsigned ord, const double start[2],
       b.tell(v);
     }
     if (found) {
-      x[0] = b.xbest_out[0];
-      x[1] = b.xbest_out[1];
-      failed = b.status == nh::BOBYQA_ROUNDOFF_LIMITED;
-      if (info) {
-        info->objective = b.fbest_out;
-        info->ord = (int)order;
-      }
+      out.x = {b.xbest_out[0], b.xbest_out[1]};
+      out.failed = b.status == nh::BOBYQA_ROUNDOFF_LIMITED;
+      if (fbest) *fbest = b.fbest_out;
     }
     if (found || order == 1) {
-      if (info) info->evaluations = evals;
-      return found;
+      out.found = found;
+      out.order = order;
+      return out;
     }
     order = 1;
   }
+}
+
+// refine_variance (nebula.rs:1972-1994): BOBYQA for both components, the
+// simplex search for one (the cell-level size held at obj.fixed_cell).
+bool refine_variance(Objective obj, unsigned ord, std::size_t n, const double start[2],
+                     const double lo[2], const double hi[2], double x[2],
+                     bool& failed, StageTwoInfo* info) {
+  auto f = [&](const std::vector<double>& xv, unsigned order) {
+    obj.params.ord = order;
+    return obj.value(xv.data());
+  };
+  double fbest = 0.0;
+  SearchOutcome r;
+  if (n == 2) {
+    r = bobyqa_search(f, ord, start, lo, hi, &fbest);
+  } else {
+    r = simplex_search(f, std::vector<double>{start[0]}, std::vector<double>{lo[0]},
+                       std::vector<double>{hi[0]}, ord);
+    // the polish moves the incumbent without evaluating it
+    fbest = std::numeric_limits<double>::quiet_NaN();
+  }
+  if (r.found) {
+    for (std::size_t j = 0; j < n; ++j) x[j] = r.x[j];
+    failed = r.failed;
+  }
+  if (info) {
+    info->evaluations = r.evaluations;
+    if (r.found) {
+      info->objective = fbest;
+      info->ord = (int)r.order;
+    }
+  }
+  return r.found;
 }
 
 struct Outcome {
@@ -504,6 +543,16 @@ bool tables_apply(const std::vector<double>& centred, std::size_t n_cells, std::
 
 }  // namespace
 
+SearchOutcome stage_two_search(const std::vector<double>& start, const std::vector<double>& lo,
+                               const std::vector<double>& hi, unsigned ord,
+                               const std::function<double(const std::vector<double>&, unsigned)>& f) {
+  if (start.size() == 2) {
+    const double s[2] = {start[0], start[1]}, l[2] = {lo[0], lo[1]}, h[2] = {hi[0], hi[1]};
+    return bobyqa_search(f, ord, s, l, h, nullptr);
+  }
+  return simplex_search(f, start, lo, hi, ord);
+}
+
 namespace {
 // When set, nebula_sparse evaluates the profile objective at these points
 // instead of fitting: (kept gene index, subject, cell) -> value.
@@ -521,12 +570,12 @@ void profile_objective(std::size_t n_genes, std::size_t n_cells, const std::uint
                        const std::uint64_t* subject_id, const double* design,
                        std::size_t n_coef, const double* offset, const Params& params,
                        std::size_t n, const std::uint64_t* gene, const double* point,
-                       double* out) {
+                       double* out, int threads) {
   ProfileProbe probe{n, gene, point, out};
   g_probe = &probe;
   try {
     nebula_sparse(n_genes, n_cells, indptr, indices, data, subject_id, design, n_coef, offset,
-                  params, 1, nullptr, nullptr);
+                  params, threads, nullptr, nullptr);
   } catch (...) {
     g_probe = nullptr;
     throw;
@@ -683,48 +732,14 @@ Fit nebula_sparse(std::size_t n_genes, std::size_t n_cells, const std::uint32_t*
 
   if (g_probe) {
     ProfileProbe* pr = g_probe;
-    for (std::size_t q = 0; q < pr->n; ++q) {
-      const std::size_t idx = pr->gene[q];
-      if (idx >= kept.size()) fail(2, "probe gene out of range");
-      const std::size_t g = kept[idx];
-      GeneCounts gc;
-      double sum = 0.0;
-      for (std::uint32_t i = indptr[g]; i < indptr[g + 1]; ++i)
-        if (data[i] > 0.0) {
-          gc.cells.push_back(indices[i]);
-          gc.counts.push_back(data[i]);
-          sum += data[i];
-          if (data[i] == 1.0) ++gc.n_one;
-          else if (data[i] == 2.0) ++gc.n_two;
-        }
-      gc.mean_count = sum / (double)n_cells;
-      const double* tot = &totals[g * k];
-      PmlData pd{sh.design, sh.log_offset, n_cells, n_coef, gc.counts.data(), gc.cells.data(),
-                 gc.counts.size(), fid.data(), k, tot};
-      std::vector<double> bs(n_coef, 0.0);
-      bs[sh.intercept] = std::log(gc.mean_count) - sh.log_mean_offset;
-      Objective obj{&pd, &bs, sh.intercept, PmlParams{}, (double)n_cells, (double)k,
-                    &gc.counts, (double)gc.counts.size(), (double)gc.n_one, (double)gc.n_two};
-      obj.params.eps = params.eps;
-      obj.params.ord = gc.mean_count * sh.cells_per_subject < kHighOrderCutoff ? 3 : 1;
-      pr->out[q] = obj.value(pr->point + 2 * q);
-    }
-    return Fit{};
-  }
-
-  std::vector<Outcome> outcomes(kept.size());
-  if (stage_two) stage_two->assign(kept.size(), StageTwoInfo{});
-  std::atomic<std::size_t> next{0};
-  std::atomic<bool> errored{false};
-  Error first_error{0, ""};
-  std::mutex mu;
-  std::atomic<long long> stage_two_ns{0};
-  auto worker = [&]() {
-    for (;;) {
-      std::size_t idx = next.fetch_add(1);
-      if (idx >= kept.size() || errored.load()) return;
-      std::size_t g = kept[idx];
-      try {
+    for (std::size_t q = 0; q < pr->n; ++q)
+      if (pr->gene[q] >= kept.size()) fail(2, "probe gene out of range");
+    std::atomic<std::size_t> next{0};
+    auto probe_worker = [&]() {
+      for (;;) {
+        const std::size_t q = next.fetch_add(1);
+        if (q >= pr->n) return;
+        const std::size_t g = kept[pr->gene[q]];
         GeneCounts gc;
         double sum = 0.0;
         for (std::uint32_t i = indptr[g]; i < indptr[g + 1]; ++i)
@@ -737,53 +752,126 @@ Fit nebula_sparse(std::size_t n_genes, std::size_t n_cells, const std::uint32_t*
           }
         gc.mean_count = sum / (double)n_cells;
         const double* tot = &totals[g * k];
-        Plan plan = plan_gene(sh, gc, tot);
         PmlData pd{sh.design, sh.log_offset, n_cells, n_coef, gc.counts.data(), gc.cells.data(),
                    gc.counts.size(), fid.data(), k, tot};
-        int refit_state = 0;
-        double v[2] = {0, 0};
-        bool failed = false;
-        StageTwoInfo info;
-        info.start_sigma = plan.sigma;
-        info.start_gamma = plan.gamma;
-        info.beta_start = plan.beta_start;
-        if (plan.refit == 2)
-          fail(2, "NEBULA-LN's one-component refit is not implemented; use method HL");
-        if (plan.refit == 1) {
-          auto t0 = std::chrono::steady_clock::now();
-          Objective obj{&pd, &plan.beta_start, sh.intercept, PmlParams{}, (double)n_cells, (double)k,
-                        &gc.counts, (double)gc.counts.size(), (double)gc.n_one, (double)gc.n_two};
-          obj.params.reml = params.reml;
-          obj.params.eps = params.eps;
-          double start[2] = {plan.sigma, plan.gamma};
-          double lo[2] = {params.min_subject, params.min_cell};
-          double hi[2] = {params.max_subject, params.max_cell};
-          bool found = refine_variance(obj, plan.ord, start, lo, hi, v, failed, &info);
-          refit_state = found ? 1 : 2;
-          info.searched = 1;
-          info.sigma = v[0];
-          info.gamma = v[1];
-          stage_two_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(
-                              std::chrono::steady_clock::now() - t0)
-                              .count();
-        }
-        outcomes[idx] = finish_gene(sh, pd, plan, refit_state, v, failed);
-        if (stage_two) (*stage_two)[idx] = info;
-      } catch (const Error& e) {
-        std::lock_guard<std::mutex> lock(mu);
-        if (!errored.exchange(true)) first_error = e;
-        return;
+        std::vector<double> bs(n_coef, 0.0);
+        bs[sh.intercept] = std::log(gc.mean_count) - sh.log_mean_offset;
+        Objective obj{&pd, &bs, sh.intercept, PmlParams{}, (double)n_cells, (double)k,
+                      &gc.counts, (double)gc.counts.size(), (double)gc.n_one, (double)gc.n_two};
+        obj.params.eps = params.eps;
+        obj.params.ord = gc.mean_count * sh.cells_per_subject < kHighOrderCutoff ? 3 : 1;
+        pr->out[q] = obj.value(pr->point + 2 * q);
       }
-    }
-  };
+    };
+    int pt = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    pt = (int)std::min<std::size_t>((std::size_t)pt, std::max<std::size_t>(pr->n, 1));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < pt; ++t) pool.emplace_back(probe_worker);
+    probe_worker();
+    for (auto& t : pool) t.join();
+    return Fit{};
+  }
+
+  // Three phases over the kept genes, each spread over the threads: stage
+  // one, stage two (timed on the wall clock, every thread busy with it), stage
+  // three.  Per gene the arithmetic is nebula.rs:1012-1033's.
+  const std::size_t n_kept = kept.size();
+  std::vector<Outcome> outcomes(n_kept);
+  std::vector<GeneCounts> gcs(n_kept);
+  std::vector<Plan> plans(n_kept);
+  std::vector<int> refit_state(n_kept, 0);
+  std::vector<std::array<double, 2>> found_v(n_kept, {0.0, 0.0});
+  std::vector<char> found_failed(n_kept, 0);
+  if (stage_two) stage_two->assign(n_kept, StageTwoInfo{});
+  std::atomic<bool> errored{false};
+  Error first_error{0, ""};
+  std::mutex mu;
   int nt = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
-  nt = (int)std::min<std::size_t>((std::size_t)nt, kept.size());
-  std::vector<std::thread> pool;
-  for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
-  worker();
-  for (auto& t : pool) t.join();
+  nt = (int)std::min<std::size_t>((std::size_t)nt, n_kept);
+  auto run_phase = [&](auto&& body) {
+    std::atomic<std::size_t> next{0};
+    auto worker = [&]() {
+      for (;;) {
+        std::size_t idx = next.fetch_add(1);
+        if (idx >= n_kept || errored.load()) return;
+        try {
+          body(idx);
+        } catch (const Error& e) {
+          std::lock_guard<std::mutex> lock(mu);
+          if (!errored.exchange(true)) first_error = e;
+          return;
+        }
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& t : pool) t.join();
+  };
+  auto pml_of = [&](std::size_t idx) {
+    const GeneCounts& gc = gcs[idx];
+    return PmlData{sh.design, sh.log_offset, n_cells, n_coef, gc.counts.data(), gc.cells.data(),
+                   gc.counts.size(), fid.data(), k, &totals[kept[idx] * k]};
+  };
+  // stage one
+  run_phase([&](std::size_t idx) {
+    const std::size_t g = kept[idx];
+    GeneCounts& gc = gcs[idx];
+    double sum = 0.0;
+    for (std::uint32_t i = indptr[g]; i < indptr[g + 1]; ++i)
+      if (data[i] > 0.0) {
+        gc.cells.push_back(indices[i]);
+        gc.counts.push_back(data[i]);
+        sum += data[i];
+        if (data[i] == 1.0) ++gc.n_one;
+        else if (data[i] == 2.0) ++gc.n_two;
+      }
+    gc.mean_count = sum / (double)n_cells;
+    plans[idx] = plan_gene(sh, gc, &totals[g * k]);
+  });
   if (errored) throw first_error;
-  if (t_stage_two_out) *t_stage_two_out = (double)stage_two_ns.load() * 1e-9;
+  // stage two
+  auto t0 = std::chrono::steady_clock::now();
+  run_phase([&](std::size_t idx) {
+    const Plan& plan = plans[idx];
+    const GeneCounts& gc = gcs[idx];
+    StageTwoInfo info;
+    info.start_sigma = plan.sigma;
+    info.start_gamma = plan.gamma;
+    info.beta_start = plan.beta_start;
+    if (plan.refit != 0) {
+      PmlData pd = pml_of(idx);
+      Objective obj{&pd, &plan.beta_start, sh.intercept, PmlParams{}, (double)n_cells, (double)k,
+                    &gc.counts, (double)gc.counts.size(), (double)gc.n_one, (double)gc.n_two};
+      obj.params.reml = params.reml;
+      obj.params.eps = params.eps;
+      double start[2] = {plan.sigma, plan.gamma};
+      double lo[2] = {params.min_subject, params.min_cell};
+      double hi[2] = {params.max_subject, params.max_cell};
+      const std::size_t n = plan.refit == 1 ? 2 : 1;
+      if (n == 1) obj.fixed_cell = plan.gamma;
+      bool failed = false;
+      bool found = refine_variance(obj, plan.ord, n, start, lo, hi, found_v[idx].data(), failed,
+                                   &info);
+      refit_state[idx] = found ? 1 : 2;
+      found_failed[idx] = failed ? 1 : 0;
+      info.searched = (int)n;
+      info.sigma = found_v[idx][0];
+      info.gamma = n == 2 ? found_v[idx][1] : plan.gamma;
+    }
+    if (stage_two) (*stage_two)[idx] = info;
+  });
+  if (errored) throw first_error;
+  const double stage_two_s =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  // stage three
+  run_phase([&](std::size_t idx) {
+    PmlData pd = pml_of(idx);
+    outcomes[idx] = finish_gene(sh, pd, plans[idx], refit_state[idx], found_v[idx].data(),
+                                found_failed[idx] != 0);
+  });
+  if (errored) throw first_error;
+  if (t_stage_two_out) *t_stage_two_out = stage_two_s;
 
   // assemble (nebula.rs:2233-2277)
   Fit fit;

@@ -10,6 +10,10 @@
 //       quoted in src/sc/pml.rs:2151-2700 (1e-11 relative)
 //   ptmg value / gradient / Hessian and the helpers     <- src/sc/ptmg.rs:2140-2898
 //   BOBYQA (shared header edge-rs_amd/csrc/bobyqa.hpp)  <- nloptr traces, exact
+//   one-component simplex search (search.hpp)          <- the reference's
+//       state-machine cases (src/sc/nebula.rs:3516-3619), scipy's Nelder-Mead
+//       on the same simplex, and the NEBULA-LN R goldens (sc, sc_blocks,
+//       sc_cat_ln)
 //   nebula_sparse end to end                            <- R nebula goldens
 //       tests/data/nebula_*.csv and tests/data/e2e/sc_*
 //   digamma / trigamma / lgamma                         <- scipy.special
@@ -20,8 +24,11 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
+
+#include "search.hpp"
 
 namespace oracle {
 
@@ -163,6 +170,12 @@ Fit nebula_sparse(std::size_t n_genes, std::size_t n_cells,
                   std::vector<StageTwoInfo>* stage_two,
                   double* t_stage_two_out);
 
+// StageTwoSearch driven by an objective f(x, order) (nebula.rs:1568-1951):
+// BOBYQA for two components, the simplex search of search.hpp for one.
+SearchOutcome stage_two_search(const std::vector<double>& start, const std::vector<double>& lo,
+                               const std::vector<double>& hi, unsigned ord,
+                               const std::function<double(const std::vector<double>&, unsigned)>& f);
+
 // The profile objective (VarianceObjective::value, nebula.rs:1489) at n
 // points (subject, cell) for kept genes gene[q], after the same set-up as
 // nebula_sparse.
@@ -171,6 +184,6 @@ void profile_objective(std::size_t n_genes, std::size_t n_cells, const std::uint
                        const std::uint64_t* subject_id, const double* design,
                        std::size_t n_coef, const double* offset, const Params& params,
                        std::size_t n, const std::uint64_t* gene, const double* point,
-                       double* out);
+                       double* out, int threads = 0);
 
 }  // namespace oracle

@@ -303,3 +303,31 @@ def profile_objective(indptr, indices, data, n_cells, subject_id, design, n_coef
     if rc != 0:
         raise OracleError(rc, err.value.decode())
     return out
+
+
+_SEARCH_FN = C.CFUNCTYPE(C.c_double, _dp, C.c_int, C.c_int, C.c_int, C.c_void_p)
+
+
+def stage_two_search(f, start, lower, upper, ord=1):
+    """StageTwoSearch on a Python objective f(x, order, call) (the reference's
+    run_search harness, src/sc/nebula.rs:3516-3542): two components run
+    BOBYQA, one the simplex search.  Returns (x or None, failed, order,
+    evaluations)."""
+    n = len(start)
+    L = lib()
+    L.oracle_stage_two_search.restype = C.c_int
+    L.oracle_stage_two_search.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _SEARCH_FN, C.c_void_p,
+                                          _dp, _ip, _ip, _ip]
+
+    def cb(x, nn, order, call, ctx):
+        return float(f(np.array([x[i] for i in range(nn)]), order, call))
+
+    fn = _SEARCH_FN(cb)
+    s, sp = _d(start)
+    lo, lop = _d(lower)
+    hi, hip = _d(upper)
+    x = np.zeros(n)
+    failed, order, evals = C.c_int(0), C.c_int(0), C.c_int(0)
+    found = L.oracle_stage_two_search(n, sp, lop, hip, ord, fn, None, x.ctypes.data_as(_dp),
+                                      C.byref(failed), C.byref(order), C.byref(evals))
+    return (x if found else None), bool(failed.value), order.value, evals.value

@@ -51,8 +51,12 @@ def golden8():
 
 
 SC_DATASETS = {
-    # tag: (method, has_offset, n_coef, design_file)
+    # tag: (method, has_offset, n_coef, design_file) -- tests/e2e_nebula.rs:312-401
+    # without sc_xwide (13 columns, beyond the device fit's 8)
+    "sc": ("LN", True, 3, False),
     "sc_small": ("LN", True, 3, False),
+    "sc_blocks": ("LN", True, 3, True),
+    "sc_cat_ln": ("LN", True, 4, True),
     "sc_k2": ("HL", True, 3, True),
     "sc_intercept": ("HL", False, 1, True),
     "sc_wide": ("HL", True, 8, True),
@@ -78,6 +82,27 @@ def load_sc(tag):
     ch, cov = read_table(f"{tag}_covariance.csv")
     return dict(counts=counts, subject=subject, design=design, offset=offset, method=method,
                 n_coef=n_coef, expected=(nh, neb), covariance=cov)
+
+
+def read_scalars():
+    """tests/golden/sc_scalars.csv as {(scenario, name): value}."""
+    out = {}
+    with open(os.path.join(GOLDEN, "sc_scalars.csv")) as fh:
+        for r in list(csv.reader(fh))[1:]:
+            if r:
+                out[(r[0], r[1])] = float(r[2])
+    return out
+
+
+def n_pinned(tag):
+    """Genes whose subject variance finishes on its lower bound, per dataset
+    (tests/e2e_nebula.rs:786-795: recorded 18 for sc and 40 for sc_small, R's
+    n_pinned scalar for the others)."""
+    if tag == "sc":
+        return 18
+    if tag == "sc_small":
+        return 40
+    return int(read_scalars()[(tag, "n_pinned")])
 
 
 def repack_r_cov(row, n):

@@ -6,7 +6,8 @@ copies are committed so the tests never read the reference at run time.
 Sources (edge-rs 0.4.0): tests/data/nebula_{counts,design,expected,covariance}.csv
 (used by src/sc/nebula.rs:2457-3187), tests/data/e2e/bobyqa_*.csv
 (src/numeric/bobyqa.rs:2947-2970), tests/data/e2e/sc_* and scalars.csv rows
-(tests/e2e_nebula.rs:288-377) for the datasets that run NEBULA-HL.
+(tests/e2e_nebula.rs:288-377), NEBULA-HL and NEBULA-LN datasets, and
+tests/data/e2e/sc_shrink.csv (tests/e2e_nebula.rs:875-930).
 """
 import os
 import shutil
@@ -18,9 +19,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 NEBULA = ["nebula_counts.csv", "nebula_design.csv", "nebula_expected.csv",
           "nebula_covariance.csv"]
 BOBYQA = ["rosen", "rosen_edge", "corner", "chain3", "rescue", "roundoff"]
-# HL datasets (sc_small is LN silently downgraded to HL at 20 cells/subject).
-SC = ["sc_small", "sc_k2", "sc_intercept", "sc_wide", "sc_high", "sc_cat_hl",
-      "sc_cat_sparse"]
+SC = ["sc", "sc_small", "sc_k2", "sc_blocks", "sc_intercept", "sc_wide", "sc_high",
+      "sc_cat_ln", "sc_cat_hl", "sc_cat_sparse"]
 
 
 def main():
@@ -38,6 +38,7 @@ def main():
             src = os.path.join(e2e, f)
             if os.path.exists(src):
                 shutil.copy(src, os.path.join(HERE, f))
+    shutil.copy(os.path.join(e2e, "sc_shrink.csv"), os.path.join(HERE, "sc_shrink.csv"))
     with open(os.path.join(e2e, "scalars.csv")) as fh, \
             open(os.path.join(HERE, "sc_scalars.csv"), "w") as out:
         for i, line in enumerate(fh):

@@ -213,3 +213,63 @@ def test_resident_submit_and_collect_reject_a_null_handle():
     assert L.nh_resident_submit(None, None, 0, 1e-6, 0, err, 256) == nh.InvalidArgument.code
     assert b"null handle" in err.value
     assert L.nh_resident_collect(None, None, 0, err, 256) == nh.InvalidArgument.code
+
+
+def _upload(gene_ptr, cells, counts, n_cells=12):
+    """nh_resident_upload through ctypes with raw arrays (None = NULL)."""
+    L = nh.lib()
+    design = np.ones(n_cells, dtype=np.float64)
+    lo = np.zeros(n_cells, dtype=np.float64)
+    ss = np.array([0, 6, n_cells], dtype=np.uint64)
+    gp = np.ascontiguousarray(gene_ptr, dtype=np.uint64)
+    ce = None if cells is None else np.ascontiguousarray(cells, dtype=np.uint32)
+    y = None if counts is None else np.ascontiguousarray(counts, dtype=np.float64)
+    h = C.c_void_p()
+    err = C.create_string_buffer(256)
+    rc = L.nh_resident_upload(n_cells, 1, 2, design.ctypes.data_as(C.POINTER(C.c_double)),
+                              lo.ctypes.data_as(C.POINTER(C.c_double)),
+                              ss.ctypes.data_as(C.POINTER(C.c_uint64)), len(gp) - 1,
+                              gp.ctypes.data_as(C.POINTER(C.c_uint64)),
+                              None if ce is None else ce.ctypes.data_as(C.POINTER(C.c_uint32)),
+                              None if y is None else y.ctypes.data_as(C.POINTER(C.c_double)),
+                              0, C.byref(h), err, 256)
+    assert not h.value, "no handle may be made for malformed input"
+    return rc, err.value.decode()
+
+
+def test_resident_upload_rejects_malformed_gene_ptr_before_any_device_call():
+    """gene_ptr must start at 0 and never decrease, and cells / counts may be
+    NULL only when no gene stores an entry: each is an error code, not a
+    device read out of bounds."""
+    cells = np.array([0, 3, 5, 7], dtype=np.uint32)
+    y = np.ones(4)
+    rc, msg = _upload([1, 4], cells, y)
+    assert rc == nh.MalformedSparse.code and "start at zero" in msg
+    rc, msg = _upload([0, 3, 1, 4], cells, y)
+    assert rc == nh.MalformedSparse.code and "non-decreasing" in msg
+    rc, msg = _upload([0, 4], None, y)
+    assert rc == nh.InvalidArgument.code and "null" in msg
+    rc, msg = _upload([0, 4], cells, None)
+    assert rc == nh.InvalidArgument.code and "null" in msg
+
+
+def test_null_indices_or_data_rejected():
+    """nebula_sparse_hip with stored entries but NULL indices or data."""
+    L = nh.lib()
+    cs, subject, design, offset = _golden_inputs()
+    for which in ("indices", "data"):
+        fit = nh._Fit()
+        err = C.create_string_buffer(256)
+        ip = np.ascontiguousarray(cs.indptr, dtype=np.uint32)
+        ix = np.ascontiguousarray(cs.indices, dtype=np.uint32)
+        dv = np.ascontiguousarray(cs.data, dtype=np.float64)
+        sid = np.ascontiguousarray(subject, dtype=np.uint64)
+        d = np.ascontiguousarray(design, dtype=np.float64)
+        rc = L.nebula_sparse_hip(cs.shape[0], cs.shape[1], ip.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                 None if which == "indices" else ix.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                 None if which == "data" else dv.ctypes.data_as(C.POINTER(C.c_double)),
+                                 sid.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                 d.ctypes.data_as(C.POINTER(C.c_double)), 3, None, None,
+                                 C.byref(fit), err, 256)
+        assert rc == nh.InvalidArgument.code, (which, rc, err.value)
+        assert b"null" in err.value

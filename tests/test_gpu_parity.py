@@ -263,21 +263,26 @@ def test_golden8_matches_r(engine):
 @pytest.mark.parametrize("tag", [t for t in SC_DATASETS if SC_DATASETS[t][2] <= 8])
 def test_sc_dataset_matches_r(tag, engine):
     """R nebula's values on the e2e datasets under the GPU path's gates
-    (tests/e2e_nebula.rs:1010-1074, per-dataset loosening :232-254); for HL
-    datasets also the stage-two objective parity against the oracle."""
+    (tests/e2e_nebula.rs:1010-1074, per-dataset loosening :232-254): NEBULA-HL
+    and NEBULA-LN (its LN+HL genes refit the subject variance alone), the
+    count of genes pinned at the variance floor, the Wald test of the HIP fit
+    against R's p and se columns; and the stage-two objective parity against
+    the oracle's own searches."""
     from test_oracle_nebula import GPU_TOLS, check_sc
     d = load_sc(tag)
     ip, ix, dv = dense_to_csr(d["counts"])
     n_cells = d["counts"].shape[1]
     fit = run_hip(ip, ix, dv, n_cells, d["subject"], d["design"], d["n_coef"],
                   d["offset"], nh.NebulaParams(method=d["method"]), ENGINES[engine])
+    print(f"{tag} {engine}: {len(fit.gene_index)} genes", flush=True)
     check_sc(tag, dict(coefficients=fit.coefficients, se=fit.se, covariance=fit.covariance,
                        subject_overdispersion=fit.subject_overdispersion,
                        cell_overdispersion=fit.cell_overdispersion, convergence=fit.convergence,
-                       gene_index=fit.gene_index), base=GPU_TOLS)
-    if d["method"] == "HL":
-        assert_objective_parity(ip, ix, dv, n_cells, d["subject"], d["design"], d["n_coef"],
-                                d["offset"], oracle.params(method="HL"), fit)
+                       gene_index=fit.gene_index, sigma_at_bound=fit.sigma_at_bound),
+             base=GPU_TOLS, wald=nh.glm_sc_test)
+    # stage two (both components, or NEBULA-LN's subject variance alone)
+    assert_objective_parity(ip, ix, dv, n_cells, d["subject"], d["design"], d["n_coef"],
+                            d["offset"], oracle.params(method=d["method"]), fit)
 
 
 @pytest.mark.parametrize("engine", list(ENGINES))
