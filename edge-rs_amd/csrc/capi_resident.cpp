@@ -12,6 +12,7 @@
 #include "persistent.hpp"
 #include "resident.hpp"
 #include "search_control.hpp"
+#include "stage_one.hpp"
 
 struct nh_resident {
   nh::Resident r;
@@ -285,6 +286,32 @@ int nh_bobyqa_device_trace(int problem, int n, const double* x0, const double* l
   }
 }
 
+int nh_stage_one_search(int n, const double* start, const double* lower, const double* upper,
+                        nh_value_grad_fn f, void* ctx, double* x_out, int* failed,
+                        int* evals_out) {
+  if (n < 1 || n > 16 || !start || !lower || !upper || !f || !x_out || !failed || !evals_out)
+    return NH_ERR_INVALID_ARGUMENT;
+  for (int i = 0; i < n; ++i)
+    if (!(lower[i] <= upper[i])) return NH_ERR_INVALID_BOUNDS;
+  std::vector<double> st(start, start + n), lo(lower, lower + n), hi(upper, upper + n);
+  nh::S1Result res;
+  nh::S1Task task = nh::stage_one_search(st, lo, hi, true, 0, &res);
+  int evals = 0;
+  while (task.waiting()) {
+    auto& pr = task.h.promise();
+    nh::EvalRes er;
+    er.g.assign(n, 0.0);
+    er.f = f(pr.req.x.data(), er.g.data(), ctx);
+    ++evals;
+    pr.res = std::move(er);
+    task.h.resume();
+  }
+  for (int i = 0; i < n; ++i) x_out[i] = res.x[i];
+  *failed = res.failed ? 1 : 0;
+  *evals_out = evals;
+  return NH_OK;
+}
+
 int nh_stage_two_search(int kind, const double* start, const double* lower, const double* upper,
                         int ord, nh_search_fn f, void* ctx, double* x_out, int* found,
                         int* failed, int* order_out, int* evals_out) {
@@ -355,6 +382,30 @@ int nh_stage_two_search_device(int problem, int kind, int ord, const double* sta
     *failed = o.failed;
     *order_out = o.ord;
     *evals_out = o.evals;
+    return NH_OK;
+  } catch (const std::exception& e) {
+    put_err(err, err_len, e.what());
+    return NH_ERR_GPU;
+  }
+}
+
+int nh_selftest_copy(size_t mib, int32_t reps, int32_t device, char* err, size_t err_len) {
+  if (mib == 0 || mib > 4096 || reps < 1 || reps > 100) {
+    put_err(err, err_len, "invalid arguments");
+    return NH_ERR_INVALID_ARGUMENT;
+  }
+  try {
+    NH_HIP(hipSetDevice(device));
+    const size_t words = mib * (size_t)(1 << 20) / 16;
+    nh::DevBuf<uint4> src, dst;
+    src.alloc(words);
+    dst.alloc(words);
+    NH_HIP(hipMemset(src.p, 0x5a, words * 16));
+    for (int r = 0; r < reps; ++r) NH_HIP(nh::launch_copy_words(src.p, dst.p, words, nullptr));
+    NH_HIP(hipDeviceSynchronize());
+    uint4 last;
+    NH_HIP(hipMemcpy(&last, dst.p + (words - 1), sizeof(last), hipMemcpyDeviceToHost));
+    if (last.x != 0x5a5a5a5au) throw std::runtime_error("copy self-test read back a wrong value");
     return NH_OK;
   } catch (const std::exception& e) {
     put_err(err, err_len, e.what());

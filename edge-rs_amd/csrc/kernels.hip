@@ -194,6 +194,21 @@ __global__ void bobyqa_trace_kernel(int problem, int n, const double* x0, const 
   result[1] = i;
 }
 
+// A plain device-to-device copy of n 16-byte words, each read and written
+// once: the yardstick the HBM byte counters are scaled by.
+__global__ void copy_words_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                  size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = src[i];
+}
+
+hipError_t launch_copy_words(const void* src, void* dst, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(copy_words_kernel, dim3(4096), dim3(256), 0, st, (const uint4*)src,
+                     (uint4*)dst, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_bobyqa_trace(int problem, int n, const double* x0, const double* lo,
                                const double* hi, double* trace, int max_trace, int* result,
                                hipStream_t st) {

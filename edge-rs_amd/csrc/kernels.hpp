@@ -60,6 +60,8 @@ hipError_t launch_batch(const BatchLaunch& L, hipStream_t st);
 hipError_t launch_f64_math(const double* x, size_t n, int which, double* out, hipStream_t st);
 // out[i] = (double)x[i]
 hipError_t launch_widen(const float* x, size_t n, double* out, hipStream_t st);
+// dst[i] = src[i] for n 16-byte words
+hipError_t launch_copy_words(const void* src, void* dst, size_t n, hipStream_t st);
 hipError_t launch_bobyqa_trace(int problem, int n, const double* x0, const double* lo,
                                const double* hi, double* trace, int max_trace, int* result,
                                hipStream_t st);

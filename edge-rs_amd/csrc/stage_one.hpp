@@ -11,15 +11,22 @@
 //   descend_marginal   nebula.rs:1132-1176 (Newton, else the quasi-Newton)
 //   newton_marginal    nebula.rs:1225-1325 (projected Newton, Levenberg shift,
 //                      Armijo halvings, speculative Hessians)
-//   quasi-Newton       projected L-BFGS with an Armijo arc search (the same
-//                      method as oracle/lbfgsb.hpp; edge-rs's own is L-BFGS-B,
-//                      src/numeric/lbfgsb.rs:876)
+//   quasi-Newton       L-BFGS-B (src/numeric/lbfgsb.rs:876-1020: compact
+//                      representation, generalised Cauchy point, subspace
+//                      minimisation, strong-Wolfe line search with bisection
+//                      zoom, memory reset on a stalled search), the middle
+//                      matrix factored once per iteration and replayed on each
+//                      right-hand side (the same operations as eliminating it
+//                      afresh for each, lbfgsb.rs:317-361)
 #pragma once
 
 #include <algorithm>
 #include <cmath>
 #include <coroutine>
+#include <cstdint>
+#include <cstring>
 #include <deque>
+#include <limits>
 #include <utility>
 #include <vector>
 
@@ -116,6 +123,272 @@ inline bool chol_solve(const std::vector<double>& a, size_t n, const std::vector
   }
   for (double v : x)
     if (!std::isfinite(v)) return false;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// L-BFGS-B's pieces that need no objective values (lbfgsb.rs:130-639).
+struct QnMemory {
+  std::vector<std::vector<double>> s, y;  // oldest first, at most kLimit pairs
+  double theta = 1.0;
+  static constexpr size_t kLimit = 10;
+  // Gaussian elimination of the middle matrix [[-D, L'], [L, theta S'S]]
+  // with partial pivoting: U in lu's upper triangle, the row factors below,
+  // the pivot of each column; usable when ok
+  std::vector<double> lu;
+  std::vector<size_t> piv;
+  bool ok = true;
+
+  size_t k() const { return s.size(); }
+  double w(size_t i, size_t j) const { return j < k() ? y[j][i] : theta * s[j - k()][i]; }
+  static double dot(const std::vector<double>& a, const std::vector<double>& b) {
+    double t = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) t += a[i] * b[i];
+    return t;
+  }
+  void reset() {
+    s.clear();
+    y.clear();
+    theta = 1.0;
+    factor();
+  }
+  void push(std::vector<double> sv, std::vector<double> yv) {
+    const double sy = dot(sv, yv), yy = dot(yv, yv);
+    if (sy <= 2.2e-16 * yy || yy <= 0.0) return;
+    if (s.size() == kLimit) {
+      s.erase(s.begin());
+      y.erase(y.begin());
+    }
+    theta = yy / sy;
+    s.push_back(std::move(sv));
+    y.push_back(std::move(yv));
+    factor();
+  }
+  void factor() {
+    const size_t kk = k(), dim = 2 * kk;
+    lu.assign(dim * dim, 0.0);
+    piv.assign(dim, 0);
+    ok = true;
+    for (size_t i = 0; i < kk; ++i) lu[i * dim + i] = -dot(s[i], y[i]);
+    for (size_t i = 0; i < kk; ++i)
+      for (size_t j = 0; j < i; ++j) {
+        const double sy = dot(s[i], y[j]);
+        lu[(kk + i) * dim + j] = sy;
+        lu[j * dim + (kk + i)] = sy;
+      }
+    for (size_t i = 0; i < kk; ++i)
+      for (size_t j = 0; j < kk; ++j) lu[(kk + i) * dim + (kk + j)] = theta * dot(s[i], s[j]);
+    for (size_t col = 0; col < dim; ++col) {
+      size_t pv = col;
+      double best = std::fabs(lu[col * dim + col]);
+      for (size_t row = col + 1; row < dim; ++row)
+        if (std::fabs(lu[row * dim + col]) > best) {
+          best = std::fabs(lu[row * dim + col]);
+          pv = row;
+        }
+      piv[col] = pv;
+      if (best < 1e-300) {
+        ok = false;
+        return;
+      }
+      if (pv != col)
+        for (size_t j = 0; j < dim; ++j) std::swap(lu[col * dim + j], lu[pv * dim + j]);
+      const double diag = lu[col * dim + col];
+      for (size_t row = col + 1; row < dim; ++row) {
+        const double f = lu[row * dim + col] / diag;
+        if (f != 0.0)
+          for (size_t j = col + 1; j < dim; ++j) lu[row * dim + j] -= f * lu[col * dim + j];
+        lu[row * dim + col] = f;  // kept for the right-hand sides
+      }
+    }
+  }
+  // v <- M v (the middle matrix's inverse applied by elimination).  The
+  // factors moved with the row swaps, so the swaps go first; each entry then
+  // takes the same updates, in the same column order, as elimination with
+  // the right-hand side alongside.
+  bool apply_m(std::vector<double>& v) const {
+    const size_t dim = 2 * k();
+    if (dim == 0) return true;
+    if (!ok) return false;
+    for (size_t col = 0; col < dim; ++col)
+      if (piv[col] != col) std::swap(v[col], v[piv[col]]);
+    for (size_t col = 0; col < dim; ++col)
+      for (size_t row = col + 1; row < dim; ++row) {
+        const double f = lu[row * dim + col];
+        if (f != 0.0) v[row] -= f * v[col];
+      }
+    for (size_t col = dim; col-- > 0;) {
+      double acc = v[col];
+      for (size_t j = col + 1; j < dim; ++j) acc -= lu[col * dim + j] * v[j];
+      v[col] = acc / lu[col * dim + col];
+    }
+    return true;
+  }
+};
+
+// Partial-pivot elimination of a dense system (lbfgsb.rs:317-361).
+inline bool qn_solve_dense(std::vector<double> m, std::vector<double>& x, size_t dim) {
+  for (size_t col = 0; col < dim; ++col) {
+    size_t pv = col;
+    double best = std::fabs(m[col * dim + col]);
+    for (size_t row = col + 1; row < dim; ++row)
+      if (std::fabs(m[row * dim + col]) > best) {
+        best = std::fabs(m[row * dim + col]);
+        pv = row;
+      }
+    if (best < 1e-300) return false;
+    if (pv != col) {
+      for (size_t j = 0; j < dim; ++j) std::swap(m[col * dim + j], m[pv * dim + j]);
+      std::swap(x[col], x[pv]);
+    }
+    const double diag = m[col * dim + col];
+    for (size_t row = col + 1; row < dim; ++row) {
+      const double f = m[row * dim + col] / diag;
+      if (f == 0.0) continue;
+      for (size_t j = col; j < dim; ++j) m[row * dim + j] -= f * m[col * dim + j];
+      x[row] -= f * x[col];
+    }
+  }
+  for (size_t col = dim; col-- > 0;) {
+    double acc = x[col];
+    for (size_t j = col + 1; j < dim; ++j) acc -= m[col * dim + j] * x[j];
+    x[col] = acc / m[col * dim + col];
+  }
+  return true;
+}
+
+inline double qn_clamp(double v, double a, double b) { return v < a ? a : (v > b ? b : v); }
+
+// f64::total_cmp's strict order
+inline bool qn_total_less(double a, double b) {
+  int64_t x, y;
+  std::memcpy(&x, &a, 8);
+  std::memcpy(&y, &b, 8);
+  x ^= (int64_t)((uint64_t)(x >> 63) >> 1);
+  y ^= (int64_t)((uint64_t)(y >> 63) >> 1);
+  return x < y;
+}
+
+// Generalised Cauchy point then subspace minimisation (Algorithm CP and the
+// direct primal method, lbfgsb.rs:400-639); false when a small system is
+// singular (the caller then takes the projected gradient step).
+inline bool qn_target(const std::vector<double>& x, const std::vector<double>& g,
+                      const std::vector<double>& lo, const std::vector<double>& hi,
+                      const QnMemory& mem, std::vector<double>& out) {
+  const size_t n = x.size(), k = mem.k(), dim = 2 * k;
+  const double inf = std::numeric_limits<double>::infinity();
+  const double eps = std::numeric_limits<double>::epsilon();
+  std::vector<double> t(n, inf), d(n, 0.0);
+  for (size_t i = 0; i < n; ++i) {
+    if (g[i] < 0.0 && std::isfinite(hi[i]))
+      t[i] = (x[i] - hi[i]) / g[i];
+    else if (g[i] > 0.0 && std::isfinite(lo[i]))
+      t[i] = (x[i] - lo[i]) / g[i];
+    d[i] = t[i] == 0.0 ? 0.0 : -g[i];
+  }
+  std::vector<double> xcp = x;
+  std::vector<char> fr(n);
+  for (size_t i = 0; i < n; ++i) fr[i] = t[i] > 0.0;
+  std::vector<size_t> order;
+  for (size_t i = 0; i < n; ++i)
+    if (t[i] > 0.0 && std::isfinite(t[i])) order.push_back(i);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t b) { return qn_total_less(t[a], t[b]); });
+  std::vector<double> c(dim, 0.0), p(dim, 0.0), mp, mc, wb(dim), mwb;
+  for (size_t j = 0; j < dim; ++j) {
+    double acc = 0.0;
+    for (size_t i = 0; i < n; ++i) acc += mem.w(i, j) * d[i];
+    p[j] = acc;
+  }
+  double dd = 0.0;
+  for (double v : d) dd += v * v;
+  double f1 = -dd;
+  mp = p;
+  if (!mem.apply_m(mp)) return false;
+  double f2 = -mem.theta * f1 - QnMemory::dot(p, mp);
+  const double f2_initial = std::max(f2, eps);
+  f2 = std::max(f2, eps);
+  double dt_min = -f1 / f2, t_old = 0.0;
+  for (size_t cur = 0; cur < order.size(); ++cur) {
+    const size_t b = order[cur];
+    const double dt = t[b] - t_old;
+    if (dt_min < dt) break;
+    xcp[b] = d[b] > 0.0 ? hi[b] : lo[b];
+    const double zb = xcp[b] - x[b];
+    fr[b] = 0;
+    for (size_t j = 0; j < dim; ++j) c[j] += dt * p[j];
+    const double gb = g[b];
+    for (size_t j = 0; j < dim; ++j) wb[j] = mem.w(b, j);
+    mc = c;
+    mp = p;
+    mwb = wb;
+    if (!mem.apply_m(mc) || !mem.apply_m(mp) || !mem.apply_m(mwb)) return false;
+    f1 += dt * f2 + gb * gb + mem.theta * gb * zb - gb * QnMemory::dot(wb, mc);
+    f2 += -mem.theta * gb * gb - 2.0 * gb * QnMemory::dot(wb, mp) - gb * gb * QnMemory::dot(wb, mwb);
+    f2 = std::max(f2, eps * f2_initial);
+    for (size_t j = 0; j < dim; ++j) p[j] += gb * wb[j];
+    d[b] = 0.0;
+    dt_min = -f1 / f2;
+    t_old = t[b];
+  }
+  dt_min = std::max(dt_min, 0.0);
+  const double t_final = t_old + dt_min;
+  for (size_t i = 0; i < n; ++i)
+    if (fr[i] && t[i] >= t_final) xcp[i] = x[i] + t_final * d[i];
+  for (size_t j = 0; j < dim; ++j) c[j] += dt_min * p[j];
+  // subspace minimisation over the free variables
+  out = xcp;
+  std::vector<size_t> free;
+  for (size_t i = 0; i < n; ++i)
+    if (fr[i]) free.push_back(i);
+  if (k == 0 || free.empty()) return true;
+  mc = c;
+  if (!mem.apply_m(mc)) return false;
+  const size_t nf = free.size();
+  std::vector<double> r(nf), v(dim, 0.0);
+  for (size_t q = 0; q < nf; ++q) {
+    const size_t i = free[q];
+    double wmc = 0.0;
+    for (size_t j = 0; j < dim; ++j) wmc += mem.w(i, j) * mc[j];
+    r[q] = g[i] + mem.theta * (xcp[i] - x[i]) - wmc;
+  }
+  for (size_t j = 0; j < dim; ++j) {
+    double acc = 0.0;
+    for (size_t q = 0; q < nf; ++q) acc += mem.w(free[q], j) * r[q];
+    v[j] = acc;
+  }
+  if (!mem.apply_m(v)) return false;
+  std::vector<double> nm(dim * dim, 0.0), col(dim);
+  for (size_t b = 0; b < dim; ++b) {
+    for (size_t a = 0; a < dim; ++a) {
+      double acc = 0.0;
+      for (size_t i : free) acc += mem.w(i, a) * mem.w(i, b);
+      col[a] = acc;
+    }
+    if (!mem.apply_m(col)) return false;
+    for (size_t a = 0; a < dim; ++a) nm[a * dim + b] = -col[a] / mem.theta;
+  }
+  for (size_t a = 0; a < dim; ++a) nm[a * dim + a] += 1.0;
+  if (!qn_solve_dense(nm, v, dim)) return false;
+  std::vector<double> dz(nf);
+  for (size_t q = 0; q < nf; ++q) {
+    double acc = 0.0;
+    for (size_t j = 0; j < dim; ++j) acc += mem.w(free[q], j) * v[j];
+    dz[q] = -(r[q] / mem.theta + acc / (mem.theta * mem.theta));
+  }
+  double alpha = 1.0;
+  for (size_t q = 0; q < nf; ++q) {
+    const size_t i = free[q];
+    if (dz[q] > 0.0 && std::isfinite(hi[i]))
+      alpha = std::min(alpha, (hi[i] - xcp[i]) / dz[q]);
+    else if (dz[q] < 0.0 && std::isfinite(lo[i]))
+      alpha = std::min(alpha, (lo[i] - xcp[i]) / dz[q]);
+  }
+  alpha = qn_clamp(alpha, 0.0, 1.0);
+  for (size_t q = 0; q < nf; ++q) {
+    const size_t i = free[q];
+    out[i] = qn_clamp(xcp[i] + alpha * dz[q], lo[i], hi[i]);
+  }
   return true;
 }
 
@@ -273,7 +546,8 @@ inline S1Task stage_one_search(std::vector<double> start, std::vector<double> lo
       }
     }
     if (!have) {
-      // projected L-BFGS from (xs, fs)
+      // L-BFGS-B from (xs, fs): minimise (lbfgsb.rs:876-1020) with memory 10,
+      // ftol relative to the start's objective (nebula.rs:1150-1156)
       const double ftol = kFtol / std::max(std::fabs(fs), 1.0);
       std::vector<double> x = xs, g;
       double f;
@@ -282,103 +556,140 @@ inline S1Task stage_one_search(std::vector<double> start, std::vector<double> lo
         f = e.f;
         g = std::move(e.g);
       }
-      std::deque<std::vector<double>> S, Y;
-      std::deque<double> R;
-      if (std::isfinite(f)) {
-        for (int it = 0; it < kLbfgsIter; ++it) {
-          double pn = 0.0;
-          for (size_t i = 0; i < n; ++i)
-            pn = std::max(pn, std::fabs(std::min(std::max(x[i] - g[i], lo[i]), hi[i]) - x[i]));
-          if (pn < kPgtol) break;
-          std::vector<char> fr(n, 1);
-          for (size_t i = 0; i < n; ++i)
-            if ((x[i] <= lo[i] && g[i] > 0.0) || (x[i] >= hi[i] && g[i] < 0.0)) fr[i] = 0;
-          std::vector<double> q(n), al(S.size()), d(n);
-          for (size_t i = 0; i < n; ++i) q[i] = fr[i] ? g[i] : 0.0;
-          for (size_t mm = S.size(); mm-- > 0;) {
-            double sq = 0.0;
-            for (size_t i = 0; i < n; ++i) sq += fr[i] ? S[mm][i] * q[i] : 0.0;
-            al[mm] = R[mm] * sq;
-            for (size_t i = 0; i < n; ++i)
-              if (fr[i]) q[i] -= al[mm] * Y[mm][i];
-          }
-          double g0s = 1.0;
-          if (!S.empty()) {
-            double sy = 0.0, yy = 0.0;
-            for (size_t i = 0; i < n; ++i) {
-              sy += S.back()[i] * Y.back()[i];
-              yy += Y.back()[i] * Y.back()[i];
-            }
-            if (yy > 0.0) g0s = sy / yy;
-          } else {
-            double gn2 = 0.0;
-            for (size_t i = 0; i < n; ++i) gn2 += fr[i] ? g[i] * g[i] : 0.0;
-            g0s = gn2 > 0.0 ? std::min(1.0, 1.0 / std::sqrt(gn2)) : 1.0;
-          }
-          for (size_t i = 0; i < n; ++i) q[i] *= g0s;
-          for (size_t mm = 0; mm < S.size(); ++mm) {
-            double yq = 0.0;
-            for (size_t i = 0; i < n; ++i) yq += fr[i] ? Y[mm][i] * q[i] : 0.0;
-            double b = R[mm] * yq;
-            for (size_t i = 0; i < n; ++i)
-              if (fr[i]) q[i] += S[mm][i] * (al[mm] - b);
-          }
-          double slope = 0.0;
-          for (size_t i = 0; i < n; ++i) {
-            d[i] = fr[i] ? -q[i] : 0.0;
-            slope += g[i] * d[i];
-          }
-          if (!(slope < 0.0)) {
-            S.clear();
-            Y.clear();
-            R.clear();
-            for (size_t i = 0; i < n; ++i) d[i] = fr[i] ? -g[i] : 0.0;
-          }
-          double stp = 1.0, fn = f;
-          std::vector<double> xn(n), gn;
-          bool acc = false;
+      QnMemory mem;
+      mem.factor();
+      for (int it = 0; it < kLbfgsIter; ++it) {
+        double pn = 0.0;
+        for (size_t i = 0; i < n; ++i)
+          pn = std::max(pn, std::fabs(qn_clamp(x[i] - g[i], lo[i], hi[i]) - x[i]));
+        if (pn <= kPgtol) break;
+        std::vector<double> target, dir(n);
+        const bool have_t = qn_target(x, g, lo, hi, mem, target);
+        bool zero = true;
+        for (size_t i = 0; i < n; ++i) {
+          dir[i] = have_t ? target[i] - x[i] : qn_clamp(x[i] - g[i], lo[i], hi[i]) - x[i];
+          zero = zero && dir[i] == 0.0;
+        }
+        if (zero) break;
+        // strong-Wolfe line search with bisection zoom (lbfgsb.rs:675-874)
+        double slope0 = 0.0;
+        for (size_t i = 0; i < n; ++i) slope0 += g[i] * dir[i];
+        double step_max = std::numeric_limits<double>::infinity();
+        for (size_t i = 0; i < n; ++i) {
+          if (dir[i] > 0.0 && std::isfinite(hi[i]))
+            step_max = std::min(step_max, (hi[i] - x[i]) / dir[i]);
+          else if (dir[i] < 0.0 && std::isfinite(lo[i]))
+            step_max = std::min(step_max, (lo[i] - x[i]) / dir[i]);
+        }
+        step_max = std::min(std::max(step_max, 0.0), 1e3);
+        bool found = false;
+        std::vector<double> xa, ga, pt(n);
+        double fa = 0.0;
+        if (slope0 < 0.0 && step_max > 0.0) {
+          int evals = 0;
+          double a_prev = 0.0, f_prev = f, a = std::min(1.0, step_max);
+          bool have_best = false, zoom = false;
+          double z_lo = 0.0, zf_lo = 0.0, z_hi = 0.0;
+          std::vector<double> xb, gb;
+          double fb = 0.0;
           for (int ls = 0; ls < kLs; ++ls) {
-            for (size_t i = 0; i < n; ++i) xn[i] = std::min(std::max(x[i] + stp * d[i], lo[i]), hi[i]);
-            double dec = 0.0;
-            for (size_t i = 0; i < n; ++i) dec += g[i] * (xn[i] - x[i]);
-            EvalRes et = co_await Eval{EvalReq{xn, false}};
-            fn = et.f;
-            if (std::isfinite(fn) && fn <= f + 1e-4 * dec) {
-              gn = std::move(et.g);
-              acc = true;
+            for (size_t i = 0; i < n; ++i) pt[i] = qn_clamp(x[i] + a * dir[i], lo[i], hi[i]);
+            EvalRes et = co_await Eval{EvalReq{pt, false}};
+            ++evals;
+            const double fn = et.f;
+            if (!std::isfinite(fn)) {
+              a *= 0.5;
+              if (a < 1e-20) break;
+              continue;
+            }
+            if (!(fn <= f + 1e-4 * a * slope0) || (ls > 0 && fn >= f_prev)) {
+              zoom = true;
+              z_lo = a_prev;
+              zf_lo = f_prev;
+              z_hi = a;
               break;
             }
-            stp *= 0.5;
-          }
-          if (!acc) break;
-          std::vector<double> s(n), y(n);
-          double sy = 0.0, yy = 0.0;
-          for (size_t i = 0; i < n; ++i) {
-            s[i] = xn[i] - x[i];
-            y[i] = gn[i] - g[i];
-            sy += s[i] * y[i];
-            yy += y[i] * y[i];
-          }
-          if (sy > 2.2e-16 * yy) {
-            S.push_back(s);
-            Y.push_back(y);
-            R.push_back(1.0 / sy);
-            if (S.size() > 10) {
-              S.pop_front();
-              Y.pop_front();
-              R.pop_front();
+            double sl = 0.0;
+            for (size_t i = 0; i < n; ++i) sl += et.g[i] * dir[i];
+            const bool curvature = std::fabs(sl) <= -0.9 * slope0;
+            if (!curvature && sl >= 0.0) {
+              zoom = true;
+              z_lo = a;
+              zf_lo = fn;
+              z_hi = a_prev;
+              break;
             }
+            // the curvature condition, or the box edge with the objective
+            // still descending
+            if (curvature || a >= step_max) {
+              found = true;
+              xa = pt;
+              fa = fn;
+              ga = std::move(et.g);
+              break;
+            }
+            have_best = true;
+            xb = pt;
+            fb = fn;
+            gb = std::move(et.g);
+            a_prev = a;
+            f_prev = fn;
+            a = std::min(a * 2.0, step_max);
           }
-          const double fold = f;
-          x = xn;
-          g = gn;
-          f = fn;
-          if (fold - f <= ftol * std::max(std::max(std::fabs(fold), std::fabs(f)), 1.0)) break;
+          if (zoom) {
+            for (int zz = evals; zz < kLs; ++zz) {
+              const double am = 0.5 * (z_lo + z_hi);
+              if (std::fabs(z_hi - z_lo) < 1e-16) break;
+              for (size_t i = 0; i < n; ++i) pt[i] = qn_clamp(x[i] + am * dir[i], lo[i], hi[i]);
+              EvalRes et = co_await Eval{EvalReq{pt, false}};
+              const double fn = et.f;
+              if (!std::isfinite(fn) || fn > f + 1e-4 * am * slope0 || fn >= zf_lo) {
+                z_hi = am;
+                continue;
+              }
+              double sl = 0.0;
+              for (size_t i = 0; i < n; ++i) sl += et.g[i] * dir[i];
+              if (std::fabs(sl) <= -0.9 * slope0) {
+                found = true;
+                xa = pt;
+                fa = fn;
+                ga = std::move(et.g);
+                break;
+              }
+              if (sl * (z_hi - z_lo) >= 0.0) z_hi = z_lo;
+              z_lo = am;
+              zf_lo = fn;
+            }
+          } else if (!found && have_best) {
+            found = true;
+            xa = xb;
+            fa = fb;
+            ga = gb;
+          }
         }
-        if (f < fs) {
-          xd = x;
-          fd = f;
+        if (!found) {
+          if (mem.k() > 0) {
+            mem.reset();
+            continue;
+          }
+          break;
         }
+        std::vector<double> sv(n), yv(n);
+        for (size_t i = 0; i < n; ++i) {
+          sv[i] = xa[i] - x[i];
+          yv[i] = ga[i] - g[i];
+        }
+        mem.push(std::move(sv), std::move(yv));
+        const double reduction = f - fa;
+        const double scale = std::max(std::max(std::fabs(f), std::fabs(fa)), 1.0);
+        x = xa;
+        f = fa;
+        g = ga;
+        if (reduction <= ftol * scale) break;
+      }
+      if (f < fs) {
+        xd = x;
+        fd = f;
       }
     }
     // ---- end of descend ----

@@ -457,6 +457,50 @@ def bobyqa_device_trace(problem, x0, lower, upper, max_trace=2000):
     return tr[: rows * (n + 1)].reshape(rows, n + 1), status
 
 
+def selftest_copy(mib=256, reps=1, device=0):
+    """A device-to-device copy of mib MiB, reps times (nh_selftest_copy): the
+    yardstick for the HBM byte counters."""
+    L = lib()
+    L.nh_selftest_copy.restype = C.c_int
+    L.nh_selftest_copy.argtypes = [C.c_size_t, C.c_int32, C.c_int32, C.c_char_p, C.c_size_t]
+    err = C.create_string_buffer(512)
+    rc = L.nh_selftest_copy(mib, reps, device, err, 512)
+    if rc != 0:
+        _raise(rc, err)
+
+
+VALUE_GRAD_FN = C.CFUNCTYPE(C.c_double, _dp, _dp, C.c_void_p)
+
+
+def stage_one_search(fg, start, lower, upper):
+    """Stage one's quasi-Newton search (bound restarts around L-BFGS-B) on the
+    host against fg(x) -> (f, g) (nh_stage_one_search).  Returns (x, failed,
+    evaluations)."""
+    n = len(start)
+
+    @VALUE_GRAD_FN
+    def cb(xp, gp, _ctx):
+        f, g = fg(np.ctypeslib.as_array(xp, (n,)).copy())
+        for i in range(n):
+            gp[i] = float(g[i])
+        return float(f)
+
+    st, stp = _d(start)
+    lo, lop = _d(lower)
+    hi, hip = _d(upper)
+    x = np.zeros(n)
+    failed, ev = C.c_int(), C.c_int()
+    L = lib()
+    L.nh_stage_one_search.restype = C.c_int
+    L.nh_stage_one_search.argtypes = [C.c_int, _dp, _dp, _dp, VALUE_GRAD_FN, C.c_void_p, _dp,
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    rc = L.nh_stage_one_search(n, stp, lop, hip, cb, None, x.ctypes.data_as(_dp),
+                               C.byref(failed), C.byref(ev))
+    if rc != 0:
+        _raise(rc, C.create_string_buffer(b"nh_stage_one_search rejected its arguments"))
+    return x, bool(failed.value), ev.value
+
+
 SEARCH_FN = C.CFUNCTYPE(C.c_double, C.c_double, C.c_double, C.c_int, C.c_void_p)
 SEARCH_BOTH = 2
 SEARCH_SUBJECT = 1

@@ -140,3 +140,38 @@ def config(i, seed=None, n_genes=None, n_cells=None):
         return generate("config5", n_genes or 20000, n_cells or 500_000, 100, 6, "lognormal1.2",
                         ("uniform", -7.0, 1.0), 5 if seed is None else seed, "generic")
     raise ValueError(i)
+
+
+def reference_bench(n_genes=200, n_cells=20_000, k=20, p=3, seed=0x5EED1234):
+    """The shape of the reference's own NEBULA bench (benches/nebula_bench.rs:
+    62-78, make_problem :136-220): balanced subjects in contiguous blocks,
+    offset ~ LogNormal(0, 0.4), design [1, subject parity, U(-1, 1)...],
+    beta0 ~ U(-2, 1.5), other coefficients ~ U(-0.5, 0.5), per-subject
+    frailty ~ Gamma(shape 4, scale 0.25), per-cell noise ~ Gamma(shape 2,
+    scale 0.5), counts ~ Poisson(offset * exp(x'beta) * frailty * noise).
+    The same distributions drawn with numpy's generator, not the reference's
+    random stream."""
+    rng = np.random.default_rng(seed)
+    sz = np.full(k, n_cells // k, dtype=np.int64)
+    sz[-1] += n_cells - sz.sum()
+    subject = np.repeat(np.arange(k, dtype=np.uint64), sz)
+    offset = rng.lognormal(0.0, 0.4, size=n_cells)
+    cols = [np.ones(n_cells), (subject % 2).astype(float)]
+    while len(cols) < p:
+        cols.append(rng.uniform(-1.0, 1.0, size=n_cells))
+    design = np.column_stack(cols)
+    indptr = np.zeros(n_genes + 1, dtype=np.uint64)
+    idx_parts, val_parts = [], []
+    for g in range(n_genes):
+        beta = np.concatenate([[rng.uniform(-2.0, 1.5)], rng.uniform(-0.5, 0.5, size=p - 1)])
+        frailty = rng.gamma(4.0, 0.25, size=k)
+        noise = rng.gamma(2.0, 0.5, size=n_cells)
+        mu = offset * np.exp(design @ beta) * np.repeat(frailty, sz) * noise
+        y = rng.poisson(np.maximum(mu, 1e-12))
+        nz = np.nonzero(y)[0]
+        idx_parts.append(nz.astype(np.uint32))
+        val_parts.append(y[nz].astype(np.float64))
+        indptr[g + 1] = indptr[g] + len(nz)
+    return Workload("reference_bench", indptr.astype(np.uint32), np.concatenate(idx_parts),
+                    np.concatenate(val_parts), n_genes, n_cells, subject,
+                    np.ascontiguousarray(design), p, offset, dict(sizes=sz))

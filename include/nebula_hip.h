@@ -312,6 +312,20 @@ int nh_bobyqa_device_trace(int problem, int n, const double* x0,
                            double* trace, int max_trace, int* evals_out,
                            char* err, size_t err_len);
 
+/* Stage one's search on the host against a caller's objective and gradient
+ * f(x, g) (minimise_marginal, src/sc/nebula.rs:1075-1116, with the
+ * quasi-Newton descent it runs where zero-count tables apply: L-BFGS-B,
+ * src/numeric/lbfgsb.rs:876, ftol 1e-6 over the start's objective, pgtol
+ * 1e-7, 300 iterations, 40 evaluations per line search, memory 10; then up
+ * to three restarts off a bound the start was inside).  The driver runs the
+ * same search with device sums for f and g.  Writes the minimiser, whether
+ * the start's value was not finite, and the evaluation count.  Returns NH_OK,
+ * NH_ERR_INVALID_ARGUMENT or NH_ERR_INVALID_BOUNDS. */
+typedef double (*nh_value_grad_fn)(const double* x, double* g, void* ctx);
+int nh_stage_one_search(int n, const double* start, const double* lower,
+                        const double* upper, nh_value_grad_fn f, void* ctx,
+                        double* x_out, int* failed, int* evals_out);
+
 /* One gene's stage-two search control, the state machine both engines run
  * (StageTwoSearch, src/sc/nebula.rs:1568-1951), on the host against a caller's
  * objective f(subject, cell, order): kind 2 searches both variance components
@@ -346,6 +360,12 @@ int nh_stage_two_search_device(int problem, int kind, int ord, const double* sta
                                double* trace, int max_trace, double* x_out, int* found,
                                int* failed, int* order_out, int* evals_out, char* err,
                                size_t err_len);
+
+/* A device-to-device copy of `mib` MiB, `reps` times (each launch reads and
+ * writes every byte once), checked by reading the last word back: the
+ * yardstick the HBM byte counters (FETCH_SIZE, WRITE_SIZE) are scaled by.
+ * No counterpart in edge-rs.  Returns NH_OK or an NH_ERR_* code. */
+int nh_selftest_copy(size_t mib, int32_t reps, int32_t device, char* err, size_t err_len);
 
 /* The FP64 sweep's exp (which 0), log (1) and reciprocal (2), and the device
  * objective's log Gamma for x > 0 (3), applied on the device to x[0..n), for

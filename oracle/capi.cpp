@@ -101,7 +101,8 @@ int oracle_profile_objective(std::size_t n_genes, std::size_t n_cells,
                              const double* data, const std::uint64_t* subject_id,
                              const double* design, std::size_t n_coef, const double* offset,
                              const nebula_params_t* p, std::size_t n, const std::uint64_t* gene,
-                             const double* point, double* out, char* err, std::size_t err_len) {
+                             const double* point, double* out, int threads, char* err,
+                             std::size_t err_len) {
   Params prm;
   if (p) {
     prm.min_subject = p->min_subject;
@@ -118,7 +119,7 @@ int oracle_profile_objective(std::size_t n_genes, std::size_t n_cells,
   }
   try {
     profile_objective(n_genes, n_cells, indptr, indices, data, subject_id, design, n_coef, offset,
-                      prm, n, gene, point, out);
+                      prm, n, gene, point, out, threads);
     return 0;
   } catch (const Error& e) {
     put_err(err, err_len, e.message);

@@ -53,6 +53,18 @@ def params(method="LN", **kw):
 _lib = None
 
 
+def default_threads():
+    """Worker threads for the oracle: OMP_NUM_THREADS where the environment
+    sets the process's CPU share, else the CPUs this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return max(1, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        return max(1, os.cpu_count() or 1)
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -195,9 +207,8 @@ _FG = C.CFUNCTYPE(C.c_double, _dp, _dp, C.c_void_p)
 
 
 def lbfgsb(fg, x0, lower, upper, ftol=1e-12, pgtol=1e-10, max_iter=500):
-    """The stage-one quasi-Newton fallback (projected L-BFGS, Armijo along the
-    projection arc; the role of src/numeric/lbfgsb.rs:876) on fg(x) -> (f, g).
-    Returns (x, f, ok)."""
+    """L-BFGS-B (src/numeric/lbfgsb.rs:876, memory 10, 40 evaluations per line
+    search) on fg(x) -> (f, g).  Returns (x, f, ok)."""
     n = len(x0)
     x0, x0p = _d(x0)
     lo, lop = _d(lower)
@@ -243,6 +254,7 @@ def nebula_sparse(indptr, indices, data, n_cells, subject_id, design, n_coef, of
     offp = None
     if offset is not None:
         off, offp = _d(offset)
+    threads = threads or default_threads()
     fit = NebulaFitC()
     info = np.zeros(8 * n_genes) if with_stage_two else None
     t2 = C.c_double(0.0)
@@ -275,7 +287,7 @@ def nebula_sparse(indptr, indices, data, n_cells, subject_id, design, n_coef, of
 
 
 def profile_objective(indptr, indices, data, n_cells, subject_id, design, n_coef, offset, prm,
-                      genes, points):
+                      genes, points, threads=0):
     """The f64 profile objective (nebula's pql_ll, -loglik) at (subject, cell)
     points for kept-gene indices `genes`."""
     n_genes = len(indptr) - 1
@@ -296,10 +308,10 @@ def profile_objective(indptr, indices, data, n_cells, subject_id, design, n_coef
     L.oracle_profile_objective.restype = C.c_int
     L.oracle_profile_objective.argtypes = [C.c_size_t, C.c_size_t, _u32p, _u32p, _dp, _u64p, _dp,
                                            C.c_size_t, _dp, C.POINTER(NebulaParams), C.c_size_t,
-                                           _u64p, _dp, _dp, C.c_char_p, C.c_size_t]
+                                           _u64p, _dp, _dp, C.c_int, C.c_char_p, C.c_size_t]
     rc = L.oracle_profile_objective(n_genes, n_cells, ipp, ixp, dvp, sidp, dep, n_coef, offp,
                                     C.byref(prm), len(g), gp, ptsp, out.ctypes.data_as(_dp),
-                                    err, 512)
+                                    threads or default_threads(), err, 512)
     if rc != 0:
         raise OracleError(rc, err.value.decode())
     return out

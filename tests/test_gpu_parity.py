@@ -144,6 +144,37 @@ def test_fp32_fit_with_fp64_finish_matches_oracle():
         assert r["finish_steps"][g] >= 1
 
 
+@pytest.mark.parametrize("cell", [1e-4, 0.5, 2.0, 1000.0])
+def test_fp32_fit_holds_across_the_variance_box(cell):
+    """The stage-two fit (FP32 Newton + FP64 finish) over the subject variances
+    the search visits, sigma^2 from 1e-4 to 1 (tests/e2e_nebula_gpu.rs:600-691:
+    subject = ln(1 + sigma^2)), at cell sizes from the box's lower corner 1e-4
+    to its upper 1000: the log-likelihood within 1e-9 of the oracle's f64
+    optimum, the log-determinant within 1e-6."""
+    w, d, lo, fid = synthetic_gene_batch(seed=17)
+    rb = nh.ResidentBatch(d, lo, fid, w.indptr.astype(np.uint64), w.indices, w.data)
+    worst_ll = worst_det = 0.0
+    for sigma2 in (1e-4, 1e-3, 1e-2, 1e-1, 1.0):
+        subject = np.log1p(sigma2)
+        genes = np.arange(w.n_genes)
+        beta0 = np.zeros((w.n_genes, w.n_coef))
+        r = rb.solve(genes, subject, cell, beta0, mode=nh.ResidentBatch.SOLVE_F32_FINISH)
+        for g in genes:
+            a, b = int(w.indptr[g]), int(w.indptr[g + 1])
+            cells = w.indices[a:b].astype(np.uint64)
+            tot = np.zeros(len(fid) - 1)
+            np.add.at(tot, np.searchsorted(fid, cells, side="right") - 1, w.data[a:b])
+            o = oracle.opt_pml(d, lo, w.data[a:b], cells, fid, tot, beta0[g], subject, cell,
+                               eps=1e-13)
+            dll = abs(r["log_likelihood"][g] - o["log_likelihood"]) / abs(o["log_likelihood"])
+            ddet = abs(r["log_det"][g] - o["log_det"]) / max(abs(o["log_det"]), 1.0)
+            worst_ll, worst_det = max(worst_ll, dll), max(worst_det, ddet)
+            assert dll <= 1e-9, (sigma2, cell, g, dll)
+            assert ddet <= 1e-6, (sigma2, cell, g, ddet)
+    print(f"cell {cell}: worst |dll| {worst_ll:.2e}, worst |dlogdet| {worst_det:.2e}")
+    rb.close()
+
+
 def test_f64_mode_matches_oracle_iterations():
     """Stage three's mode (f64 opt_pml, ord 3) against the oracle on 6000-cell
     genes: same iteration count, log-likelihood to 1e-12, and the quantities

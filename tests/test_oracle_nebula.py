@@ -194,8 +194,21 @@ def check_sc(tag, fit, base=BASE, wald=oracle_wald.glm_sc_test):
     cov = np.asarray(fit["covariance"])
     for j, name in enumerate(names):
         w = wald(np.asarray(fit["coefficients"]), cov, coef=j)
-        assert_close(w["p_value"][pure], neb[pure, col["p_" + name]], *tol["p_value"],
-                     label=f"{tag}/wald_p_{name}")
+        got_p, want_p = w["p_value"][pure], neb[pure, col["p_" + name]]
+        if base is GPU_TOLS:
+            # A p-value's relative error is about z^2 times z's, so at the GPU
+            # path's 1e-2-class SE gates it is held to TOL_P_VALUE where
+            # p >= 1e-10 (z < 6.4) and, deeper in the tail, on the log scale:
+            # log p ~ -z^2 / 2 carries twice z's relative error.
+            body = want_p >= 1e-10
+            if body.any():
+                assert_close(got_p[body], want_p[body], *tol["p_value"],
+                             label=f"{tag}/wald_p_{name}")
+            if (~body).any():
+                assert_close(np.log(got_p[~body]), np.log(want_p[~body]), 1e-3,
+                             label=f"{tag}/wald_log_p_{name}")
+        else:
+            assert_close(got_p, want_p, *tol["p_value"], label=f"{tag}/wald_p_{name}")
         assert_close(w["se"][pure], neb[pure, col["se_" + name]], *tol["se"],
                      label=f"{tag}/wald_se_{name}")
 
@@ -206,12 +219,7 @@ def run_sc(tag, runner):
     return runner(ip, ix, dv, d)
 
 
-@pytest.mark.parametrize("tag", [
-    pytest.param(t, marks=pytest.mark.xfail(
-        strict=True, reason="stage one's quasi-Newton is a projected L-BFGS, not the "
-        "reference's L-BFGS-B: on this categorical design a pure-LN gene (stage one only) "
-        "stops 4e-6 from R's point and one Wald p-value of 1e-54 misses 1e-2 by 4 %"))
-    if t == "sc_cat_ln" else t for t in SC_DATASETS])
+@pytest.mark.parametrize("tag", list(SC_DATASETS))
 def test_sc_dataset_matches_r(tag):
     def runner(ip, ix, dv, d):
         prm = oracle.params(method=d["method"])
