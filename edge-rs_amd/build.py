@@ -23,6 +23,7 @@ SOURCES = [
     "stage_kernels.hip",
     "persistent.hip",
     "mlp.hip",
+    "noalloc.hip",
     "resident.cpp",
     "capi_resident.cpp",
     "driver.cpp",

@@ -18,6 +18,11 @@ struct nh_resident {
   nh::Resident r;
 };
 
+namespace nh {
+// The handle's resident batch, for the entry points defined in other files.
+Resident* resident_of(nh_resident_t* h) { return &h->r; }
+}  // namespace nh
+
 namespace {
 void put_err(char* err, size_t len, const std::string& m) {
   if (err && len) {

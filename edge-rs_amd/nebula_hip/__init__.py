@@ -350,10 +350,8 @@ class ResidentBatch:
         self.submit(genes, subject, cell, beta_init, log_w_init, ord, eps, mode)
         return self.collect()
 
-    def submit(self, genes, subject, cell, beta_init, log_w_init=None, ord=1, eps=1e-6,
-               mode=SOLVE_F32_FINISH):
-        """Launches the fits and returns at once (nh_resident_submit); one
-        launch may be in flight, and collect() returns its replies."""
+    def _requests(self, genes, subject, cell, beta_init, log_w_init, ord):
+        """nh_request_t array and the numpy arrays its pointers view."""
         n = len(genes)
         nb, k = self.n_coef, self.k
         bi = np.ascontiguousarray(np.broadcast_to(np.asarray(beta_init, dtype=np.float64),
@@ -373,32 +371,23 @@ class ResidentBatch:
             reqs[q].cell = float(cell[q])
             reqs[q].beta_init = bi[q].ctypes.data_as(_dp)
             reqs[q].log_w_init = lw[q].ctypes.data_as(_dp) if lw is not None else None
-        err = C.create_string_buffer(512)
-        rc = lib().nh_resident_submit(self._h, reqs, n, eps, mode, err, 512)
-        if rc != 0:
-            _raise(rc, err)
-        self._pending = n
+        return reqs, (bi, lw)
 
-    def collect(self, n=None):
-        """Waits for the launch in flight and returns its replies as a dict of
-        arrays (nh_resident_collect); n defaults to the submitted count."""
-        n = getattr(self, "_pending", 0) if n is None else n
+    def _replies(self, n):
         nb, k = self.n_coef, self.k
-        beta = np.zeros((n, nb))
-        logw = np.zeros((n, k))
-        info = np.zeros((n, nb * nb))
+        arrays = dict(beta=np.zeros((n, nb)), log_w=np.zeros((n, k)),
+                      information=np.zeros((n, nb * nb)))
         reps = (_Reply * n)()
         for q in range(n):
-            reps[q].beta = beta[q].ctypes.data_as(_dp)
-            reps[q].log_w = logw[q].ctypes.data_as(_dp)
-            reps[q].information = info[q].ctypes.data_as(_dp)
-        err = C.create_string_buffer(512)
-        rc = lib().nh_resident_collect(self._h, reps, n, err, 512)
-        if rc != 0:
-            _raise(rc, err)
-        self._pending = 0
+            reps[q].beta = arrays["beta"][q].ctypes.data_as(_dp)
+            reps[q].log_w = arrays["log_w"][q].ctypes.data_as(_dp)
+            reps[q].information = arrays["information"][q].ctypes.data_as(_dp)
+        return reps, arrays
+
+    @staticmethod
+    def _reply_dict(reps, arrays):
         return dict(
-            beta=beta, log_w=logw, information=info,
+            **arrays,
             log_likelihood=np.array([r.log_likelihood for r in reps]),
             log_likelihood_prev=np.array([r.log_likelihood_prev for r in reps]),
             log_det=np.array([r.log_det for r in reps]),
@@ -407,6 +396,46 @@ class ResidentBatch:
             backtracks=np.array([r.backtracks for r in reps]),
             finish_steps=np.array([r.finish_steps for r in reps]),
             convergence=np.array([r.convergence for r in reps]))
+
+    def submit(self, genes, subject, cell, beta_init, log_w_init=None, ord=1, eps=1e-6,
+               mode=SOLVE_F32_FINISH):
+        """Launches the fits and returns at once (nh_resident_submit); one
+        launch may be in flight, and collect() returns its replies."""
+        reqs, _keep = self._requests(genes, subject, cell, beta_init, log_w_init, ord)
+        err = C.create_string_buffer(512)
+        rc = lib().nh_resident_submit(self._h, reqs, len(genes), eps, mode, err, 512)
+        if rc != 0:
+            _raise(rc, err)
+        self._pending = len(genes)
+
+    def collect(self, n=None):
+        """Waits for the launch in flight and returns its replies as a dict of
+        arrays (nh_resident_collect); n defaults to the submitted count."""
+        n = getattr(self, "_pending", 0) if n is None else n
+        reps, arrays = self._replies(n)
+        err = C.create_string_buffer(512)
+        rc = lib().nh_resident_collect(self._h, reps, n, err, 512)
+        if rc != 0:
+            _raise(rc, err)
+        self._pending = 0
+        return self._reply_dict(reps, arrays)
+
+    def solve_noalloc(self, genes, subject, cell, beta_init, log_w_init=None, ord=1, eps=1e-6,
+                      mode=SOLVE_F32_FINISH):
+        """solve() through nh_resident_solve_noalloc: the same fits with every
+        device buffer in the library's fixed per-device scratch array."""
+        reqs, _keep = self._requests(genes, subject, cell, beta_init, log_w_init, ord)
+        reps, arrays = self._replies(len(genes))
+        L = lib()
+        L.nh_resident_solve_noalloc.restype = C.c_int
+        L.nh_resident_solve_noalloc.argtypes = [C.c_void_p, C.POINTER(_Request), C.c_size_t,
+                                                C.c_double, C.c_int32, C.POINTER(_Reply),
+                                                C.c_char_p, C.c_size_t]
+        err = C.create_string_buffer(512)
+        rc = L.nh_resident_solve_noalloc(self._h, reqs, len(genes), eps, mode, reps, err, 512)
+        if rc != 0:
+            _raise(rc, err)
+        return self._reply_dict(reps, arrays)
 
     def close(self):
         if getattr(self, "_h", None):

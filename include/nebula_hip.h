@@ -293,6 +293,21 @@ int nh_resident_solve(nh_resident_t* r, const nh_request_t* requests,
                       size_t n_req, double eps, int32_t mode,
                       nh_reply_t* replies, char* err, size_t err_len);
 
+/* nh_resident_solve without allocation, for callers that cannot allocate
+ * device memory (serving loops, code under stream capture): same arguments,
+ * same kernels, same replies.  No counterpart in edge-rs.  Every device
+ * buffer beyond the handle's resident data (requests, start points, replies,
+ * result arrays, the fit's global-memory work area, its counters) lives in
+ * one file-scope device array of 32 MiB per GPU, held until the process ends;
+ * a batch needing more is NH_ERR_INVALID_ARGUMENT (about 250k requests at 4
+ * coefficients and 20 subjects, ~300 at 8 coefficients and 180 subjects).
+ * Host staging uses fixed stack buffers; no hipMalloc, hipFree or other
+ * allocating routine is called.  Calls that overlap on one GPU share the
+ * array and race; the array keeps the last call's contents. */
+int nh_resident_solve_noalloc(nh_resident_t* r, const nh_request_t* requests,
+                              size_t n_req, double eps, int32_t mode,
+                              nh_reply_t* replies, char* err, size_t err_len);
+
 void nh_resident_free(nh_resident_t* r);
 
 /* BOBYQA as the device and the host search run it (NLopt 2.7.1 semantics).
