@@ -3,6 +3,12 @@ the C ABI in include/nebula_hip.h) into edge-rs_amd/nebula_hip/.
 
     python edge-rs_amd/build.py            # full rebuild
     python edge-rs_amd/build.py --incremental
+    python edge-rs_amd/build.py --sanitize  # host code under ASan + UBSan, into
+                                            # edge-rs_amd/_build_asan/libnebula_hip.so
+
+The sanitizer build instruments host code only (each -fsanitize= follows
+-Xarch_host; device code is built as usual).  Load it with NH_LIBRARY pointing
+at it and the clang ASan runtime preloaded (tests/run_sanitized.sh).
 
 Every HIP source is compiled with -packed-fp32-ops disabled (no v_pk_*_f32).
 """
@@ -38,6 +44,10 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
           "-munsafe-fp-atomics"]
 
 
+SANITIZE = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-fno-sanitize-recover=all"]
+
+
 def newest_dep_mtime():
     m = 0.0
     for f in os.listdir(CSRC):
@@ -47,15 +57,15 @@ def newest_dep_mtime():
     return m
 
 
-def compile_one(src, incremental):
+def compile_one(src, incremental, build=BUILD, extra=()):
     path = os.path.join(CSRC, src)
-    obj = os.path.join(BUILD, src + ".o")
+    obj = os.path.join(build, src + ".o")
     if incremental and os.path.exists(obj):
         if os.path.getmtime(obj) >= max(os.path.getmtime(path), newest_dep_mtime()):
             return obj, 0.0
-    cmd = [HIPCC] + COMMON + DEVICE + ["-c", path, "-o", obj]
+    cmd = [HIPCC] + COMMON + DEVICE + list(extra) + ["-c", path, "-o", obj]
     if src.endswith(".cpp"):
-        cmd = [HIPCC] + COMMON + DEVICE + ["-x", "hip", "-c", path, "-o", obj]
+        cmd = [HIPCC] + COMMON + DEVICE + list(extra) + ["-x", "hip", "-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
@@ -63,25 +73,32 @@ def compile_one(src, incremental):
     return obj, 1.0
 
 
-def build(incremental=False, jobs=8):
-    os.makedirs(BUILD, exist_ok=True)
-    os.makedirs(os.path.dirname(OUT), exist_ok=True)
+def build(incremental=False, jobs=8, sanitize=False):
+    build_dir = os.path.join(HERE, "_build_asan") if sanitize else BUILD
+    out = os.path.join(build_dir, "libnebula_hip.so") if sanitize else OUT
+    extra = SANITIZE if sanitize else []
+    os.makedirs(build_dir, exist_ok=True)
+    os.makedirs(os.path.dirname(out), exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda s: compile_one(s, incremental)[0], srcs))
-    tmp = OUT + ".tmp"
+        objs = list(ex.map(lambda s: compile_one(s, incremental, build_dir, extra)[0], srcs))
+    tmp = out + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"]
+    if sanitize:
+        cmd += ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+                "-shared-libasan"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link failed")
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--incremental", action="store_true")
     ap.add_argument("-j", type=int, default=8)
+    ap.add_argument("--sanitize", action="store_true")
     a = ap.parse_args()
-    print(build(a.incremental, a.j))
+    print(build(a.incremental, a.j, a.sanitize))

@@ -16,7 +16,9 @@ from typing import Optional
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libnebula_hip.so")
+# NH_LIBRARY selects another build of the library (the sanitizer build of
+# edge-rs_amd/build.py --sanitize); the in-tree build otherwise
+LIB_PATH = os.environ.get("NH_LIBRARY") or os.path.join(HERE, "libnebula_hip.so")
 
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)

@@ -411,3 +411,24 @@ def test_many_subjects_work_arrays_in_global_memory(engine):
     vc = oracle.profile_objective(w.indptr, w.indices, w.data, w.n_cells, w.subject_id,
                                   w.design, w.n_coef, w.offset, prm, g, ref["stage_two"][:, 1:3])
     assert np.max(np.abs(v - vc) / np.abs(vc)) < 1e-6
+
+
+@pytest.mark.parametrize("engine", list(ENGINES))
+def test_at_bound_code_matches_oracle(engine):
+    """check_convergence's -60 (a variance component on its upper subject or
+    lower cell bound after stage two, src/sc/pml.rs:1111-1139 called from
+    finish_gene, src/sc/nebula.rs:966-968): with the subject variance capped
+    at 0.02 on sc_small, the genes the oracle reports at -60 are the HIP
+    path's, gene for gene."""
+    d = load_sc("sc_small")
+    ip, ix, dv = dense_to_csr(d["counts"])
+    n_cells = d["counts"].shape[1]
+    fit = run_hip(ip, ix, dv, n_cells, d["subject"], d["design"], d["n_coef"], d["offset"],
+                  nh.NebulaParams(method="LN", max=(0.02, 1000.0)), ENGINES[engine])
+    ref = oracle.nebula_sparse(ip, ix, dv, n_cells, d["subject"], d["design"], d["n_coef"],
+                               d["offset"], oracle.params(method="LN", max=(0.02, 1000.0)))
+    assert np.array_equal(fit.gene_index, ref["gene_index"])
+    want = ref["convergence"] == -60
+    assert want.sum() > 20
+    np.testing.assert_array_equal(fit.convergence == -60, want)
+    assert np.all(fit.subject_overdispersion[want] == 0.02)

@@ -38,3 +38,36 @@ def test_gradient_matches_finite_differences():
         pm[base + i] -= h
         fd = (mlp.loss_grad(pp, f, t)[0] - mlp.loss_grad(pm, f, t)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, fd, g[i])
+
+
+def test_forward_and_gradient_match_torch():
+    """The oracle's forward pass, Huber loss and gradient against torch's own
+    layers in float64 (nn.functional.linear, gelu(approximate='tanh'),
+    huber_loss with delta 1 and autograd): a wrong GELU constant, a transposed
+    weight or a dropped bias fails here, which finite differences of the
+    oracle's own loss cannot show."""
+    import torch
+    rng = np.random.default_rng(11)
+    p = mlp.random_params(7)
+    f = rng.normal(size=(13, 12))
+    t = rng.normal(size=(13, 2))
+    q = mlp.unpack(p)
+    tq = {k: torch.tensor(v, dtype=torch.float64, requires_grad=k in dict(mlp.SHAPES))
+          for k, v in q.items()}
+    x = (torch.tensor(f) - tq["fm"]) / tq["fs"]
+    a = x
+    for i in (1, 2, 3):
+        a = torch.nn.functional.gelu(torch.nn.functional.linear(a, tq[f"W{i}"], tq[f"b{i}"]),
+                                     approximate="tanh")
+    out = torch.nn.functional.linear(a, tq["W4"], tq["b4"])
+    want_out, _ = mlp.forward(p, f)
+    np.testing.assert_allclose(out.detach().numpy(), want_out, rtol=1e-13, atol=1e-13)
+    target = (torch.tensor(t) - torch.tensor(f[:, mlp.BASE:mlp.BASE + mlp.OUT]) - tq["tm"]) / tq["ts"]
+    loss = torch.nn.functional.huber_loss(out, target, reduction="sum", delta=1.0) / len(f)
+    loss.backward()
+    want_loss, want_grad = mlp.loss_grad(p, f, t)
+    np.testing.assert_allclose(loss.item(), want_loss, rtol=1e-13)
+    got = np.concatenate([tq[name].grad.numpy().ravel() for name, _ in mlp.SHAPES])
+    np.testing.assert_allclose(got, want_grad, rtol=1e-11, atol=1e-14)
+    np.testing.assert_allclose(mlp.predict(p, f), out.detach().numpy() * q["ts"] + q["tm"]
+                               + f[:, mlp.BASE:mlp.BASE + mlp.OUT], rtol=1e-13, atol=1e-13)
