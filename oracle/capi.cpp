@@ -43,8 +43,9 @@ int oracle_nebula_sparse(std::size_t n_genes, std::size_t n_cells, const std::ui
                          const std::uint64_t* subject_id, const double* design,
                          std::size_t n_coef, const double* offset, const nebula_params_t* p,
                          int threads, nebula_fit_t* out, double* stage_two_info,
-                         double* t_stage_two, char* err, std::size_t err_len) {
+                         double* t_stage_two, int tables, char* err, std::size_t err_len) {
   Params prm;
+  prm.tables = tables != 0;
   if (p) {
     prm.min_subject = p->min_subject;
     prm.min_cell = p->min_cell;
@@ -101,9 +102,10 @@ int oracle_profile_objective(std::size_t n_genes, std::size_t n_cells,
                              const double* data, const std::uint64_t* subject_id,
                              const double* design, std::size_t n_coef, const double* offset,
                              const nebula_params_t* p, std::size_t n, const std::uint64_t* gene,
-                             const double* point, double* out, int threads, char* err,
-                             std::size_t err_len) {
+                             const double* point, double* out, int threads, int tables,
+                             char* err, std::size_t err_len) {
   Params prm;
+  prm.tables = tables != 0;
   if (p) {
     prm.min_subject = p->min_subject;
     prm.min_cell = p->min_cell;
@@ -169,10 +171,18 @@ int oracle_opt_pml(const double* design, const double* offset, std::size_t n_cel
                    std::size_t nnz, const std::uint64_t* start, std::size_t k,
                    const double* totals, const double* beta_init, const double* log_w_init,
                    double subject, double cell, int reml, double eps, int ord, int max_iter,
-                   int nbm, double* beta, double* log_w, double* information,
+                   int nbm, int tabled, double* beta, double* log_w, double* information,
                    double* scalars, int* ints, char* err, std::size_t err_len) {
   std::vector<std::size_t> c, s;
   PmlData d = make_pml(design, offset, n_cells, nb, counts, cells, nnz, start, k, totals, c, s);
+  ZeroCells zc;
+  if (tabled) {
+    if (!build_zero_cells(design, offset, n_cells, s.data(), k, nb, zc)) {
+      put_err(err, err_len, "zero-count tables do not apply to this design");
+      return 3;
+    }
+    d.zeros = &zc;
+  }
   PmlParams p;
   p.reml = reml != 0;
   p.eps = eps;
@@ -217,15 +227,65 @@ int oracle_pml_ll_grad(const double* design, const double* offset, std::size_t n
 int oracle_ptmg(const double* design, const double* log_offset, std::size_t n_cells,
                 std::size_t n_coef, const double* counts, const std::uint64_t* cells,
                 std::size_t nnz, const double* totals, const std::uint64_t* fid, std::size_t k,
-                const double* params, double* value, double* grad, double* hess) {
+                const double* params, int tabled, double* value, double* grad, double* hess) {
   std::vector<std::size_t> c(cells, cells + nnz), f(fid, fid + k + 1);
   GeneData g{design, log_offset, n_cells, n_coef, counts, c.data(), nnz, totals, f.data(), k};
+  ZeroCells zc;
+  if (tabled) {
+    if (!build_zero_cells(design, log_offset, n_cells, f.data(), k, n_coef, zc)) return 3;
+    g.zeros = &zc;
+  }
   Ptmg pt(g);
   if (hess)
     *value = pt.value_gradient_hessian(params, grad, hess);
   else
     *value = pt.value_gradient(params, grad);
   return 0;
+}
+
+// One zero-count group built from log offsets log_o[n]: its table evaluated
+// at ns points (m sums from `from`, row-major ns x m), the compensated direct
+// sums there (ns x 6) when `direct` is given, and (s_lo, s_hi, n_base,
+// n_panels) in info[4].
+void oracle_zero_group(const double* log_o, std::size_t n, const double* s, std::size_t ns,
+                       int from, int m, double* out, double* direct, double* info) {
+  std::vector<double> lo(log_o, log_o + n);
+  ZeroGroup g = build_zero_group(0, lo);
+  for (std::size_t i = 0; i < ns; ++i) g.eval(from, m, s[i], out + i * (std::size_t)m);
+  if (direct) {
+    std::vector<std::pair<double, double>> distinct;
+    for (double l : lo) distinct.push_back({l, 1.0});
+    for (std::size_t i = 0; i < ns; ++i) zero_direct_sums(distinct, s[i], direct + i * 6);
+  }
+  if (info) {
+    info[0] = g.s_lo;
+    info[1] = g.s_hi;
+    info[2] = (double)g.n_base;
+    info[3] = (double)(g.panels.size() / (6 * 17));
+  }
+}
+
+// ZeroCells::build on a design: 1 when built, 0 when the tables do not apply.
+// Outputs (when built): subject_groups[k + 1], subject_loose[k + 1],
+// loose[n_cells] (the first subject_loose[k] entries), group_row and group_n
+// (n_groups entries, at most n_cells).
+int oracle_zero_cells(const double* design, const double* log_offset, std::size_t n_cells,
+                      const std::uint64_t* fid, std::size_t k, std::size_t n_coef,
+                      std::uint64_t* subject_groups, std::uint64_t* subject_loose,
+                      std::uint64_t* loose, std::uint64_t* group_row, double* group_n) {
+  std::vector<std::size_t> f(fid, fid + k + 1);
+  ZeroCells z;
+  if (!build_zero_cells(design, log_offset, n_cells, f.data(), k, n_coef, z)) return 0;
+  for (std::size_t s = 0; s <= k; ++s) {
+    subject_groups[s] = z.subject_groups[s];
+    subject_loose[s] = z.subject_loose[s];
+  }
+  for (std::size_t i = 0; i < z.loose.size(); ++i) loose[i] = z.loose[i];
+  for (std::size_t i = 0; i < z.groups.size(); ++i) {
+    group_row[i] = z.groups[i].row;
+    group_n[i] = z.groups[i].n;
+  }
+  return 1;
 }
 
 void oracle_special(int which, const double* x, double* y, std::size_t n) {

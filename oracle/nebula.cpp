@@ -12,10 +12,9 @@
 // assemble                   <- nebula.rs:2233-2277
 // Helpers from src/sc/ptmg.rs: offset_summary :1827, centre_design :1748,
 // cell_level_columns :1962, design_cv :1888, cumsum_y :1596,
-// positive_indices :1681.  Zero-count tables (src/sc/zeros.rs) are not built:
-// their sums equal the dense sweep to ~1e-13 (zeros.rs:512-597), so only
-// their side effect is kept, the choice of the quasi-Newton for stage one
-// when they would apply (zeros.rs:219-262's 32-cell / 50% rule).
+// positive_indices :1681.  Zero-count tables (zeros.cpp, src/sc/zeros.rs) are
+// built as nebula.rs:641 builds them and used by stage one (quasi-Newton on
+// the tabled marginal likelihood) and by every penalised fit.
 // NEBULA-LN's one-component refit (Nelder-Mead plus polish, nebula.rs:1798-1950)
 // is restated in search.hpp.
 //
@@ -131,7 +130,8 @@ struct Shared {
   std::vector<std::size_t> cell_columns;
   int method;
   Params params;
-  bool tables;  // zero-count tables would apply
+  bool tables;                 // zero-count tables apply (zeros.rs:219-262)
+  const ZeroCells* zeros;      // the tables, when they apply and are used
 };
 
 struct GeneCounts {
@@ -324,7 +324,7 @@ Plan plan_gene(const Shared& sh, const GeneCounts& gc, const double* totals) {
   Plan plan;
   plan.ord = gc.mean_count * sh.cells_per_subject < kHighOrderCutoff ? 3 : 1;
   GeneData gd{sh.design, sh.log_offset, sh.n_cells, nb, gc.counts.data(), gc.cells.data(),
-              gc.counts.size(), totals, sh.fid, sh.k};
+              gc.counts.size(), totals, sh.fid, sh.k, sh.zeros};
   Ptmg pt(gd);
   double lmc = std::log(gc.mean_count);
   std::vector<double> start(nb + 2, 0.0), lo(nb + 2, -kBetaBound), hi(nb + 2, kBetaBound);
@@ -523,23 +523,6 @@ Outcome finish_gene(const Shared& sh, const PmlData& pd, const Plan& plan, int r
   return o;
 }
 
-// ZeroCells::build's decision (zeros.rs:219-262): tables apply when cells in
-// within-subject groups of >= 32 identical design rows make up half the run.
-bool tables_apply(const std::vector<double>& centred, std::size_t n_cells, std::size_t nb,
-                  const std::vector<std::size_t>& fid) {
-  std::size_t tabled = 0;
-  for (std::size_t s = 0; s + 1 < fid.size(); ++s) {
-    std::map<std::vector<std::uint64_t>, std::size_t> groups;
-    for (std::size_t r = fid[s]; r < fid[s + 1]; ++r) {
-      std::vector<std::uint64_t> key(nb);
-      std::memcpy(key.data(), &centred[r * nb], nb * sizeof(double));
-      ++groups[key];
-    }
-    for (auto& kv : groups)
-      if (kv.second >= 32) tabled += kv.second;
-  }
-  return !((double)tabled < 0.5 * (double)n_cells);
-}
 
 }  // namespace
 
@@ -704,7 +687,13 @@ Fit nebula_sparse(std::size_t n_genes, std::size_t n_cells, const std::uint32_t*
   }
   sh.method = sh.cells_per_subject < kMinCellsPerSubjectLn ? 1 : params.method;
   sh.params = params;
-  sh.tables = tables_apply(centred, n_cells, n_coef, fid);
+  // zero-count tables from the centred design and the log offsets
+  // (nebula.rs:641); params.tables = false sweeps every cell instead, the
+  // check that both give the same fits
+  ZeroCells zc;
+  sh.tables = build_zero_cells(centred.data(), log_offset.data(), n_cells, fid.data(), k,
+                               n_coef, zc);
+  sh.zeros = sh.tables && params.tables ? &zc : nullptr;
 
   // cumsum_y and the expression filter (nebula.rs:621-636)
   std::vector<std::size_t> subject_of(n_cells);
@@ -753,7 +742,7 @@ Fit nebula_sparse(std::size_t n_genes, std::size_t n_cells, const std::uint32_t*
         gc.mean_count = sum / (double)n_cells;
         const double* tot = &totals[g * k];
         PmlData pd{sh.design, sh.log_offset, n_cells, n_coef, gc.counts.data(), gc.cells.data(),
-                   gc.counts.size(), fid.data(), k, tot};
+                   gc.counts.size(), fid.data(), k, tot, sh.zeros};
         std::vector<double> bs(n_coef, 0.0);
         bs[sh.intercept] = std::log(gc.mean_count) - sh.log_mean_offset;
         Objective obj{&pd, &bs, sh.intercept, PmlParams{}, (double)n_cells, (double)k,
@@ -811,7 +800,7 @@ Fit nebula_sparse(std::size_t n_genes, std::size_t n_cells, const std::uint32_t*
   auto pml_of = [&](std::size_t idx) {
     const GeneCounts& gc = gcs[idx];
     return PmlData{sh.design, sh.log_offset, n_cells, n_coef, gc.counts.data(), gc.cells.data(),
-                   gc.counts.size(), fid.data(), k, &totals[kept[idx] * k]};
+                   gc.counts.size(), fid.data(), k, &totals[kept[idx] * k], sh.zeros};
   };
   // stage one
   run_phase([&](std::size_t idx) {

@@ -16,6 +16,11 @@
 //       sc_cat_ln)
 //   nebula_sparse end to end                            <- R nebula goldens
 //       tests/data/nebula_*.csv and tests/data/e2e/sc_*
+//   zero-count tables (zeros.cpp)                       <- their own compensated
+//       direct sums over the whole range (1e-13, zeros.rs:512-535), numpy and
+//       finite differences for the sums, zeros.rs:543-557's grouping case, and
+//       the tabled fit / marginal likelihood against the dense ones
+//       (pml.rs:2926-2960)
 //   digamma / trigamma / lgamma                         <- scipy.special
 //   quasi-Newton (stage-one fallback)                   <- the reference's
 //       L-BFGS-B known-answer problems (src/numeric/lbfgsb.rs:1040-1296) and
@@ -44,6 +49,33 @@ double ln_gamma(double x);
 double digamma(double x);
 double trigamma(double x);
 
+// ---- sc/zeros.rs ----
+// One subject's cells sharing a design row: the six sums over them at a
+// count of zero, L = sum ln(1 + v) and its five derivatives in s = ln(tau)
+// (v = tau O), as piecewise Chebyshev interpolants.
+struct ZeroGroup {
+  std::size_t row = 0;  // a member cell (the first)
+  double n = 0, sum_o = 0, sum_inv_o = 0, sum_log_o = 0, s_lo = 0, s_hi = 0, lo = 0;
+  std::size_t n_base = 0;
+  std::vector<std::pair<std::uint32_t, std::uint8_t>> index;  // first panel, depth
+  std::vector<double> panels;  // per panel: 6 forms x 17 coefficients
+  // m sums from sum `from` (0 L .. 5 E); C, D, E need B in the same call
+  void eval(int from, int m, double s, double* out) const;
+};
+
+struct ZeroCells {
+  std::vector<ZeroGroup> groups;
+  std::vector<std::size_t> subject_groups;  // k + 1
+  std::vector<std::size_t> loose;           // cells in no group, ascending per subject
+  std::vector<std::size_t> subject_loose;   // k + 1
+};
+
+ZeroGroup build_zero_group(std::size_t row, const std::vector<double>& log_o);
+bool build_zero_cells(const double* design, const double* log_offset, std::size_t n_cells,
+                      const std::size_t* fid, std::size_t k, std::size_t n_coef, ZeroCells& z);
+void zero_direct_sums(const std::vector<std::pair<double, double>>& distinct, double s,
+                      double* out);
+
 // ---- sc/pml.rs ----
 struct PmlData {
   const double* design;         // n_cells * nb, row-major
@@ -56,6 +88,7 @@ struct PmlData {
   const std::size_t* subject_start;  // k + 1
   std::size_t k;
   const double* subject_total;  // k
+  const ZeroCells* zeros = nullptr;  // tables built from this design, or none
 };
 
 struct PmlParams {
@@ -101,6 +134,7 @@ struct GeneData {
   const double* subject_totals;
   const std::size_t* fid;
   std::size_t k;
+  const ZeroCells* zeros = nullptr;
 };
 
 class Ptmg {
@@ -114,6 +148,7 @@ class Ptmg {
 
  private:
   double linear_predictor(const double* beta);
+  double value_gradient_tabled(const double* params, double* grad);
   GeneData g_;
   std::vector<double> y_cell_, extb_, cumsumxtb_;
 };
@@ -140,6 +175,7 @@ struct Params {
   double cutoff_cell = 20.0, kappa = 800.0, cpc = 0.005;
   std::size_t mincp = 5, min_subjects = 0;
   double eps = 1e-6;
+  bool tables = true;  // false: sweep every cell even where tables apply
 };
 
 struct Fit {

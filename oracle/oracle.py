@@ -75,20 +75,26 @@ def lib():
         L.oracle_nebula_sparse.argtypes = [C.c_size_t, C.c_size_t, _u32p, _u32p, _dp, _u64p,
                                            _dp, C.c_size_t, _dp, C.POINTER(NebulaParams),
                                            C.c_int, C.POINTER(NebulaFitC), _dp, _dp,
-                                           C.c_char_p, C.c_size_t]
+                                           C.c_int, C.c_char_p, C.c_size_t]
         L.oracle_fit_free.argtypes = [C.POINTER(NebulaFitC)]
         L.oracle_opt_pml.restype = C.c_int
         L.oracle_opt_pml.argtypes = [_dp, _dp, C.c_size_t, C.c_size_t, _dp, _u64p, C.c_size_t,
                                      _u64p, C.c_size_t, _dp, _dp, _dp, C.c_double, C.c_double,
-                                     C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, _dp, _dp,
-                                     _dp, _dp, _ip, C.c_char_p, C.c_size_t]
+                                     C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     _dp, _dp, _dp, _dp, _ip, C.c_char_p, C.c_size_t]
         L.oracle_pml_ll_grad.restype = C.c_int
         L.oracle_pml_ll_grad.argtypes = [_dp, _dp, C.c_size_t, C.c_size_t, _dp, _u64p,
                                          C.c_size_t, _u64p, C.c_size_t, _dp, _dp, _dp,
                                          C.c_double, C.c_double, _dp, _dp]
         L.oracle_ptmg.restype = C.c_int
         L.oracle_ptmg.argtypes = [_dp, _dp, C.c_size_t, C.c_size_t, _dp, _u64p, C.c_size_t,
-                                  _dp, _u64p, C.c_size_t, _dp, _dp, _dp, _dp]
+                                  _dp, _u64p, C.c_size_t, _dp, C.c_int, _dp, _dp, _dp]
+        L.oracle_zero_group.restype = None
+        L.oracle_zero_group.argtypes = [_dp, C.c_size_t, _dp, C.c_size_t, C.c_int, C.c_int, _dp,
+                                        _dp, _dp]
+        L.oracle_zero_cells.restype = C.c_int
+        L.oracle_zero_cells.argtypes = [_dp, _dp, C.c_size_t, _u64p, C.c_size_t, C.c_size_t,
+                                        _u64p, _u64p, _u64p, _u64p, _dp]
         L.oracle_special.argtypes = [C.c_int, _dp, _dp, C.c_size_t]
         L.oracle_min_eigenvalue.restype = C.c_double
         L.oracle_min_eigenvalue.argtypes = [_dp, C.c_size_t]
@@ -122,7 +128,7 @@ class OracleError(RuntimeError):
 
 def opt_pml(design, offset, counts, cells, subject_start, subject_total, beta_init,
             subject, cell, reml=False, eps=1e-6, ord=1, log_w_init=None, max_iter=0,
-            nbm=False):
+            nbm=False, tabled=False):
     """oracle opt_pml (src/sc/pml.rs:468) on one gene; returns a dict."""
     design = np.asarray(design, dtype=np.float64)
     n_cells = len(offset)
@@ -150,7 +156,7 @@ def opt_pml(design, offset, counts, cells, subject_start, subject_total, beta_in
     err = C.create_string_buffer(512)
     rc = lib().oracle_opt_pml(args[0], args[1], n_cells, nb, args[2], cp, len(c), sp, k, tp,
                               bp, lwp, subject, cell, int(reml), eps, ord, max_iter, int(nbm),
-                              beta.ctypes.data_as(_dp), log_w.ctypes.data_as(_dp),
+                              int(tabled), beta.ctypes.data_as(_dp), log_w.ctypes.data_as(_dp),
                               info.ctypes.data_as(_dp), sc.ctypes.data_as(_dp),
                               ints.ctypes.data_as(_ip), err, 512)
     if rc != 0:
@@ -183,7 +189,7 @@ def pml_ll_grad(design, offset, counts, cells, subject_start, subject_total, bet
     return obj.value, grad
 
 
-def ptmg(design, log_offset, counts, cells, totals, fid, params, hessian=True):
+def ptmg(design, log_offset, counts, cells, totals, fid, params, hessian=True, tabled=False):
     """Stage-one marginal NLL, gradient and Hessian (src/sc/ptmg.rs:585,634)."""
     design = np.asarray(design, dtype=np.float64)
     n = len(log_offset)
@@ -198,9 +204,54 @@ def ptmg(design, log_offset, counts, cells, totals, fid, params, hessian=True):
     v = C.c_double()
     g = np.zeros(nb + 2)
     h = np.zeros((nb + 2) ** 2) if hessian else None
-    lib().oracle_ptmg(dp, op, n, nb, yp, cp, len(c), tp, fp, len(f) - 1, xp, C.byref(v),
-                      g.ctypes.data_as(_dp), h.ctypes.data_as(_dp) if hessian else None)
+    rc = lib().oracle_ptmg(dp, op, n, nb, yp, cp, len(c), tp, fp, len(f) - 1, xp, int(tabled),
+                           C.byref(v), g.ctypes.data_as(_dp),
+                           h.ctypes.data_as(_dp) if hessian else None)
+    if rc:
+        raise OracleError(rc, "zero-count tables do not apply to this design")
     return v.value, g, h
+
+
+def zero_group(log_o, s, frm=0, m=6, direct=True):
+    """One zero-count group (src/sc/zeros.rs:291) from log offsets: the table
+    at points s (len(s) x m), the compensated direct sums (len(s) x 6) and
+    (s_lo, s_hi, n_base, n_panels)."""
+    lo, lop = _d(log_o)
+    sv, sp = _d(np.atleast_1d(s))
+    out = np.zeros(len(sv) * m)
+    dirs = np.zeros(len(sv) * 6) if direct else None
+    info = np.zeros(4)
+    lib().oracle_zero_group(lop, len(lo), sp, len(sv), frm, m, out.ctypes.data_as(_dp),
+                            dirs.ctypes.data_as(_dp) if direct else None,
+                            info.ctypes.data_as(_dp))
+    return (out.reshape(len(sv), m), dirs.reshape(len(sv), 6) if direct else None, info)
+
+
+def zero_cells(design, log_offset, fid):
+    """ZeroCells::build (src/sc/zeros.rs:219): None when the tables do not
+    apply, else a dict of subject_groups, loose, subject_loose, group_row and
+    group_n."""
+    design = np.asarray(design, dtype=np.float64)
+    n = len(log_offset)
+    nb = design.size // n
+    d, dp = _d(design)
+    o, op = _d(log_offset)
+    f, fp = _u64(fid)
+    k = len(f) - 1
+    sg = np.zeros(k + 1, dtype=np.uint64)
+    sl = np.zeros(k + 1, dtype=np.uint64)
+    lo = np.zeros(n, dtype=np.uint64)
+    gr = np.zeros(n, dtype=np.uint64)
+    gn = np.zeros(n)
+    ok = lib().oracle_zero_cells(dp, op, n, fp, k, nb, sg.ctypes.data_as(_u64p),
+                                 sl.ctypes.data_as(_u64p), lo.ctypes.data_as(_u64p),
+                                 gr.ctypes.data_as(_u64p), gn.ctypes.data_as(_dp))
+    if not ok:
+        return None
+    ng = int(sg[-1])
+    return dict(subject_groups=sg.astype(np.int64), subject_loose=sl.astype(np.int64),
+                loose=lo[: int(sl[-1])].astype(np.int64), group_row=gr[:ng].astype(np.int64),
+                group_n=gn[:ng].copy())
 
 
 _FG = C.CFUNCTYPE(C.c_double, _dp, _dp, C.c_void_p)
@@ -242,7 +293,7 @@ def special(which, x):
 
 
 def nebula_sparse(indptr, indices, data, n_cells, subject_id, design, n_coef, offset=None,
-                  prm=None, threads=0, with_stage_two=False):
+                  prm=None, threads=0, with_stage_two=False, tables=True):
     """oracle nebula_sparse (src/sc/nebula.rs:434) -> dict of numpy arrays."""
     n_genes = len(indptr) - 1
     ip, ipp = _u32(indptr)
@@ -263,7 +314,7 @@ def nebula_sparse(indptr, indices, data, n_cells, subject_id, design, n_coef, of
                                     C.byref(prm) if prm is not None else None, threads,
                                     C.byref(fit),
                                     info.ctypes.data_as(_dp) if info is not None else None,
-                                    C.byref(t2), err, 1024)
+                                    C.byref(t2), int(tables), err, 1024)
     if rc != 0:
         raise OracleError(rc, err.value.decode())
     g = fit.n_genes
@@ -287,7 +338,7 @@ def nebula_sparse(indptr, indices, data, n_cells, subject_id, design, n_coef, of
 
 
 def profile_objective(indptr, indices, data, n_cells, subject_id, design, n_coef, offset, prm,
-                      genes, points, threads=0):
+                      genes, points, threads=0, tables=True):
     """The f64 profile objective (nebula's pql_ll, -loglik) at (subject, cell)
     points for kept-gene indices `genes`."""
     n_genes = len(indptr) - 1
@@ -308,10 +359,11 @@ def profile_objective(indptr, indices, data, n_cells, subject_id, design, n_coef
     L.oracle_profile_objective.restype = C.c_int
     L.oracle_profile_objective.argtypes = [C.c_size_t, C.c_size_t, _u32p, _u32p, _dp, _u64p, _dp,
                                            C.c_size_t, _dp, C.POINTER(NebulaParams), C.c_size_t,
-                                           _u64p, _dp, _dp, C.c_int, C.c_char_p, C.c_size_t]
+                                           _u64p, _dp, _dp, C.c_int, C.c_int, C.c_char_p,
+                                           C.c_size_t]
     rc = L.oracle_profile_objective(n_genes, n_cells, ipp, ixp, dvp, sidp, dep, n_coef, offp,
                                     C.byref(prm), len(g), gp, ptsp, out.ctypes.data_as(_dp),
-                                    threads or default_threads(), err, 512)
+                                    threads or default_threads(), int(tables), err, 512)
     if rc != 0:
         raise OracleError(rc, err.value.decode())
     return out

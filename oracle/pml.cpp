@@ -2,7 +2,8 @@
 //
 // optimise        <- pml.rs:1614-1800 (Newton over (beta, log w) through the
 //                    Schur complement, per-coordinate backtracking, codes)
-// sweep           <- pml.rs:1419-1590 (dense pass at a count of zero, then the
+// sweep           <- pml.rs:1419-1590 (dense pass at a count of zero, or the
+//                    zero-count tables' groups plus the loose cells, then the
 //                    positive counts; exact because every term is linear in y)
 // laplace_corr.   <- pml.rs:1828-1919
 // ldlt_solve      <- pml.rs:1940-2021 (Eigen-style pivoted LDL')
@@ -84,6 +85,7 @@ void sweep_width(const PmlData& d, const double* beta, const double* log_w,
   std::fill(out.db.begin(), out.db.end(), 0.0);
   std::fill(out.vb.begin(), out.vb.end(), 0.0);
   double ll = 0.0;
+  const double ln_gamma = std::log(gamma);
   std::vector<double> lw(log_w, log_w + k);
   for (std::size_t s = 0; s < k; ++s) {
     const std::size_t c0 = d.subject_start[s], c1 = d.subject_start[s + 1];
@@ -105,19 +107,49 @@ void sweep_width(const PmlData& d, const double* beta, const double* log_w,
         for (std::size_t b = a; b < nb; ++b) second[a * nb + b] += pa * row[b];
       }
     };
-    for (std::size_t r = c0; r < c1; ++r) {
-      const double* row = d.design + r * nb;
-      double eta = d.offset[r];
-      for (std::size_t j = 0; j < nb; ++j) eta += row[j] * beta[j];
-      stored[r] = std::exp(eta + lws);
-    }
-    for (std::size_t r = c0; r < c1; ++r) {
-      double e = stored[r];
-      double t = e + gamma;
-      sum_log += std::log(t);
-      double inv = 1.0 / t;
-      double u = e * inv;
-      fold(r, -gamma * u, gamma * u * inv);
+    if (!d.zeros) {
+      for (std::size_t r = c0; r < c1; ++r) {
+        const double* row = d.design + r * nb;
+        double eta = d.offset[r];
+        for (std::size_t j = 0; j < nb; ++j) eta += row[j] * beta[j];
+        stored[r] = std::exp(eta + lws);
+      }
+      for (std::size_t r = c0; r < c1; ++r) {
+        double e = stored[r];
+        double t = e + gamma;
+        sum_log += std::log(t);
+        double inv = 1.0 / t;
+        double u = e * inv;
+        fold(r, -gamma * u, gamma * u * inv);
+      }
+    } else {
+      // per group at a count of zero: sum ln t = n ln gamma + L, sum u = A,
+      // sum gamma u / t = B at ln tau = log w + x'beta - ln gamma
+      // (pml.rs:1510-1536); then the loose cells one by one
+      const ZeroCells& z = *d.zeros;
+      for (std::size_t gi = z.subject_groups[s]; gi < z.subject_groups[s + 1]; ++gi) {
+        const ZeroGroup& g = z.groups[gi];
+        const double* row = d.design + g.row * nb;
+        double log_tau = lws - ln_gamma;
+        for (std::size_t j = 0; j < nb; ++j) log_tau += row[j] * beta[j];
+        double lab[3];
+        g.eval(0, 3, log_tau, lab);
+        sum_log += g.n * ln_gamma + lab[0];
+        fold(g.row, -gamma * lab[1], lab[2]);
+      }
+      for (std::size_t li = z.subject_loose[s]; li < z.subject_loose[s + 1]; ++li) {
+        const std::size_t r = z.loose[li];
+        const double* row = d.design + r * nb;
+        double eta = d.offset[r];
+        for (std::size_t j = 0; j < nb; ++j) eta += row[j] * beta[j];
+        const double e = std::exp(eta + lws);
+        stored[r] = e;
+        const double t = e + gamma;
+        sum_log += std::log(t);
+        const double inv = 1.0 / t;
+        const double u = e * inv;
+        fold(r, -gamma * u, gamma * u * inv);
+      }
     }
     for (std::size_t i = run[s]; i < run[s + 1]; ++i) {
       std::size_t c = d.cells[i];
@@ -125,7 +157,13 @@ void sweep_width(const PmlData& d, const double* beta, const double* log_w,
       const double* row = d.design + c * nb;
       double eta = d.offset[c];
       for (std::size_t j = 0; j < nb; ++j) eta += row[j] * beta[j];
-      double e = stored[c];
+      double e;
+      if (d.zeros) {
+        e = std::exp(eta + lws);
+        stored[c] = e;
+      } else {
+        e = stored[c];
+      }
       double t = e + gamma;
       linear += eta * y;
       wlog += y * std::log(t);
@@ -180,10 +218,11 @@ void validate(const PmlData& d) {
 
 // Higher Laplace terms (pml.rs:1828-1919).
 double laplace_correction(const PmlData& d, const std::vector<double>& stored,
-                          const std::vector<std::size_t>& run,
-                          const std::vector<double>& w, double gamma,
+                          const std::vector<std::size_t>& run, const double* beta,
+                          const double* log_w, const std::vector<double>& w, double gamma,
                           double lambda, unsigned ord) {
   const bool high = ord > 2;
+  const double ln_gamma = std::log(gamma);
   auto terms = [&](double e, double out[4]) {
     double t = e + gamma;
     double p2 = e / (t * t);
@@ -200,9 +239,26 @@ double laplace_correction(const PmlData& d, const std::vector<double>& stored,
   double third_acc = 0.0, fourth_acc = 0.0, fifth_acc = 0.0;
   for (std::size_t s = 0; s < d.k; ++s) {
     double sums[4] = {0, 0, 0, 0}, f[4];
-    for (std::size_t r = d.subject_start[s]; r < d.subject_start[s + 1]; ++r) {
-      terms(stored[r], f);
-      for (int j = 0; j < 4; ++j) sums[j] += gamma * f[j];
+    if (!d.zeros) {
+      for (std::size_t r = d.subject_start[s]; r < d.subject_start[s + 1]; ++r) {
+        terms(stored[r], f);
+        for (int j = 0; j < 4; ++j) sums[j] += gamma * f[j];
+      }
+    } else {
+      // loose cells, then each group's B..E (gamma times each factor)
+      const ZeroCells& z = *d.zeros;
+      for (std::size_t li = z.subject_loose[s]; li < z.subject_loose[s + 1]; ++li) {
+        terms(stored[z.loose[li]], f);
+        for (int j = 0; j < 4; ++j) sums[j] += gamma * f[j];
+      }
+      for (std::size_t gi = z.subject_groups[s]; gi < z.subject_groups[s + 1]; ++gi) {
+        const ZeroGroup& g = z.groups[gi];
+        const double* row = d.design + g.row * d.nb;
+        double log_tau = log_w[s] - ln_gamma;
+        for (std::size_t j = 0; j < d.nb; ++j) log_tau += row[j] * beta[j];
+        g.eval(2, 4, log_tau, f);
+        for (int j = 0; j < 4; ++j) sums[j] += f[j];
+      }
     }
     for (std::size_t i = run[s]; i < run[s + 1]; ++i) {
       terms(stored[d.cells[i]], f);
@@ -335,7 +391,8 @@ PmlResult optimise(const PmlData& d, const double* beta_init,
   }
   double second = 0.0;
   if (pen.gamma && params.ord > 1)
-    second = laplace_correction(d, stored, run, cur.w, gamma, pen.lambda, params.ord);
+    second = laplace_correction(d, stored, run, beta.data(), log_w.data(), cur.w, gamma,
+                                pen.lambda, params.ord);
 
   PmlResult r;
   r.beta = beta;

@@ -6,10 +6,12 @@
 // gamma_terms    <- ptmg.rs:404-478
 // value+gradient <- ptmg.rs:661-896 (linear_predictor, Subjects, evaluate,
 //                   gradient_parts, finish_gradient)
+// tabled value+gradient <- ptmg.rs:1462-1580 (zero counts from the tables)
 // Hessian        <- ptmg.rs:999-1252 (evaluate_hessian) with the per-subject
 //                   sums of hessian_subject (:1310-1444)
 // Pinned by the nebula 1.5.8 values quoted in ptmg.rs:2163-2445 and by finite
 // differences (tests/test_oracle_ptmg.py).
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -185,7 +187,110 @@ double Ptmg::value(const double* params) {
   return value_gradient(params, grad.data());
 }
 
+// Value and gradient with the zero counts summed from the run's tables
+// (ptmg.rs:1462-1580): per group, sum ym e is ym exp(x'beta) sum O exactly and
+// the log terms are L and A at s = ln(ym) + x'beta - ln(gamma); loose and
+// positive cells one by one.
+double Ptmg::value_gradient_tabled(const double* params, double* grad_out) {
+  const std::size_t nb = g_.n_coef, k = g_.k;
+  const ZeroCells& z = *g_.zeros;
+  const double* beta = params;
+  Terms t(params[nb], params[nb + 1]);
+  const double gamma = t.gamma, ln_gamma = t.log_gamma;
+  auto row_of = [&](std::size_t r) { return g_.design + r * nb; };
+  auto dot = [&](const double* row) {
+    double acc = 0.0;
+    for (std::size_t j = 0; j < nb; ++j) acc += row[j] * beta[j];
+    return acc;
+  };
+  // positive entries of each subject: [run[s], run[s + 1])
+  std::vector<std::size_t> run(k + 1);
+  for (std::size_t s = 0; s <= k; ++s)
+    run[s] = std::lower_bound(g_.cells, g_.cells + g_.nnz, g_.fid[s]) - g_.cells;
+  double total = 0.0;
+  std::vector<double> group_xb(z.groups.size(), 0.0), xf(k * nb, 0.0);
+  for (std::size_t s = 0; s < k; ++s) {
+    double* xs = &xf[s * nb];
+    double csx = 0.0;
+    for (std::size_t gi = z.subject_groups[s]; gi < z.subject_groups[s + 1]; ++gi) {
+      const ZeroGroup& g = z.groups[gi];
+      const double* row = row_of(g.row);
+      const double xb = dot(row);
+      group_xb[gi] = xb;
+      const double e = std::exp(xb) * g.sum_o;
+      csx += e;
+      for (std::size_t j = 0; j < nb; ++j) xs[j] += row[j] * e;
+    }
+    for (std::size_t li = z.subject_loose[s]; li < z.subject_loose[s + 1]; ++li) {
+      const std::size_t r = z.loose[li];
+      const double* row = row_of(r);
+      const double e = std::exp(g_.log_offset[r] + dot(row));
+      extb_[r] = e;
+      csx += e;
+      for (std::size_t j = 0; j < nb; ++j) xs[j] += row[j] * e;
+    }
+    for (std::size_t i = run[s]; i < run[s + 1]; ++i) {
+      const std::size_t c = g_.cells[i];
+      const double eta = g_.log_offset[c] + dot(row_of(c));
+      total += eta * g_.counts[i];
+      extb_[c] = std::exp(eta);
+    }
+    cumsumxtb_[s] = csx;
+  }
+  Subjects sj = subjects(g_, t, cumsumxtb_, total);
+  double slpey = 0.0, gstar_sum = 0.0;
+  std::vector<double> d41(k * nb, 0.0), d42c(k, 0.0);
+  for (std::size_t s = 0; s < k; ++s) {
+    const double ym = sj.ym[s];
+    const double ln_ym = std::log(ym);
+    double* ds = &d41[s * nb];
+    double acc = 0.0;
+    total += ym * cumsumxtb_[s];
+    for (std::size_t gi = z.subject_groups[s]; gi < z.subject_groups[s + 1]; ++gi) {
+      const ZeroGroup& g = z.groups[gi];
+      const double* row = row_of(g.row);
+      double la[2];
+      g.eval(0, 2, ln_ym + group_xb[gi] - ln_gamma, la);
+      const double sum_log_w = g.n * ln_gamma + la[0];
+      total -= gamma * sum_log_w;
+      slpey += sum_log_w;
+      gstar_sum += g.n - la[1];
+      const double ge = gamma / ym * la[1];  // sum gamma e / w
+      acc += ge;
+      for (std::size_t j = 0; j < nb; ++j) ds[j] += row[j] * ge;
+    }
+    auto cell = [&](std::size_t r, double y, bool dense) {
+      const double e = extb_[r];
+      const double w = ym * e + gamma;
+      const double lw = std::log(w);
+      const double gw = dense ? gamma / w : y / w;
+      if (dense) {
+        total -= gamma * lw;
+        slpey += lw;
+      } else {
+        total -= y * lw;
+      }
+      gstar_sum += gw;
+      const double ge = gw * e;
+      acc += ge;
+      const double* row = row_of(r);
+      for (std::size_t j = 0; j < nb; ++j) ds[j] += row[j] * ge;
+    };
+    for (std::size_t li = z.subject_loose[s]; li < z.subject_loose[s + 1]; ++li)
+      cell(z.loose[li], 0.0, true);
+    for (std::size_t i = run[s]; i < run[s + 1]; ++i) cell(g_.cells[i], g_.counts[i], false);
+    d42c[s] = acc - cumsumxtb_[s];
+  }
+  GradParts gp = gradient_parts(g_, t, sj, xf, d41, d42c, slpey, gstar_sum);
+  GammaTerms ex = gamma_terms(g_, t.alpha, gamma, 1);
+  for (std::size_t j = 0; j < nb + 2; ++j) grad_out[j] = -gp.grad[j];
+  grad_out[nb] -= t.alpha_pr * ex.sd;
+  grad_out[nb + 1] -= ex.cd;
+  return -total - ex.sv - ex.cv;
+}
+
 double Ptmg::value_gradient(const double* params, double* grad_out) {
+  if (g_.zeros) return value_gradient_tabled(params, grad_out);
   const std::size_t nb = g_.n_coef, k = g_.k;
   Terms t(params[nb], params[nb + 1]);
   const double gamma = t.gamma;
