@@ -34,6 +34,7 @@ SOURCES = [
     "capi_resident.cpp",
     "driver.cpp",
     "sc_test.cpp",
+    "zero_tables.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

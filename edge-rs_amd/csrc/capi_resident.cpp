@@ -58,6 +58,7 @@ void nebula_hip_options_default(nebula_hip_options_t* o) {
   o->host_threads = 0;
   o->learned_step = 0.0;
   o->learned_xtol = 0.0;
+  o->zero_tables = 1;
 }
 
 int nh_resident_upload(size_t n_cells, size_t n_coef, size_t n_subjects, const double* design,

@@ -527,38 +527,16 @@ Prepared prepare(size_t n_genes, size_t n_cells, const uint32_t* indptr, const u
   P.intercept = constant[0];
   P.cells_per_subject = (double)n_cells / (double)P.k;
   P.method = P.cells_per_subject < 30.0 ? NH_METHOD_HL : p.method;
-  // zero-count tables would apply (src/sc/zeros.rs:219-262): stage one then
-  // runs the quasi-Newton, as edge-rs does.
+  // zero-count tables (src/sc/zeros.rs:219-278, built as nebula.rs:641 builds
+  // them): where groups of >= 32 cells sharing a design row within a subject
+  // hold half the cells, every device sweep reads them, and stage one runs
+  // the quasi-Newton, as edge-rs does
   {
-    // cells per distinct design row within each subject (rows compared bit
-    // for bit, at most 8 columns)
-    using Row = std::array<uint64_t, 8>;
-    struct RowHash {
-      size_t operator()(const Row& r) const {
-        uint64_t h = 0x9e3779b97f4a7c15ull;
-        for (uint64_t w : r) {
-          h ^= w + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
-          h *= 0xbf58476d1ce4e5b9ull;
-        }
-        return (size_t)(h ^ (h >> 31));
-      }
-    };
-    // subjects in parallel, each with its own map
-    std::vector<size_t> tabled_s(P.k, 0);
-    parallel_for(P.k, 0, [&](size_t s) {
-      std::unordered_map<Row, size_t, RowHash> groups;
-      groups.reserve(P.fid[s + 1] - P.fid[s]);
-      for (size_t r = P.fid[s]; r < P.fid[s + 1]; ++r) {
-        Row key{};
-        std::memcpy(key.data(), &P.centred[r * n_coef], n_coef * sizeof(double));
-        ++groups[key];
-      }
-      for (auto& kv : groups)
-        if (kv.second >= 32) tabled_s[s] += kv.second;
-    });
-    size_t tabled = 0;
-    for (size_t t : tabled_s) tabled += t;
-    P.tables = !((double)tabled < 0.5 * (double)n_cells);
+    const auto tz = std::chrono::steady_clock::now();
+    P.tables = build_zero_tables(P.centred.data(), P.log_offset.data(), n_cells, n_coef,
+                                 reinterpret_cast<const uint64_t*>(P.fid.data()), P.k, 0, P.zt);
+    P.t_tables =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - tz).count();
   }
   // cell-level columns (ptmg.rs:1962-2011)
   for (size_t j = 0; j < n_coef; ++j) {
@@ -1106,6 +1084,7 @@ int nh_session_create(size_t n_genes, size_t n_cells, const uint32_t* indptr,
     S->R.upload(n_cells, n_coef, S->P.k, S->P.centred.data(), S->P.log_offset.data(),
                 reinterpret_cast<const uint64_t*>(S->P.fid.data()), S->P.kept.size(),
                 S->P.gene_ptr.data(), S->P.cells.data(), S->P.counts.data(), S->o.device);
+    if (S->P.tables && S->o.zero_tables) S->R.set_tables(S->P.zt);
     const auto t2 = Clock::now();
     run_stage_one(S->P, S->R, S->o.host_threads, S->s1_rounds);
     compute_features(S->P);
@@ -1218,6 +1197,8 @@ int nh_session_stage_two(nh_session_t* s, int32_t engine, double* minimisers,
     st.t_upload = s->t_upload;
     st.t_stage_one = s->t_stage_one;
     st.t_stage_two = secs(t3, t4);
+    st.zero_groups = s->R.tabled ? s->R.zgroup.n : 0;
+    st.t_tables = s->P.t_tables;
     for (size_t i = 0; i < s->P.plans.size(); ++i) {
       const GenePlan& pl = s->P.plans[i];
       const bool ok = pl.searched && pl.found;

@@ -1,6 +1,8 @@
 // Per-run host state shared by the driver's stages.
 #pragma once
 
+#include "zero_tables.hpp"
+
 #include <cstdint>
 #include <vector>
 
@@ -39,7 +41,9 @@ struct Prepared {
   std::vector<double> log_offset, centred, sds;
   double log_mean_offset = 0, cv2 = 0, cells_per_subject = 0;
   int method = NH_METHOD_HL;
-  bool tables = false;
+  bool tables = false;  // zero-count tables apply (and are in zt)
+  ZeroTables zt;
+  double t_tables = 0.0;  // seconds spent building them
   std::vector<size_t> cell_columns, kept;
   // kept genes' positive counts, CSR over kept genes
   std::vector<uint64_t> gene_ptr;

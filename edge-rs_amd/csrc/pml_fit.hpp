@@ -47,6 +47,7 @@
 #include <stdint.h>
 
 #include "device_common.hpp"
+#include "zero_tables.hpp"
 
 namespace nh {
 
@@ -61,7 +62,15 @@ struct DeviceData {
   const float* gy32;     // positive counts
   const double* gy64;
   const double* gtot;    // n_genes * k subject totals
-  uint32_t n_cells, nb, k, icpt, rw;
+  // zero-count tables (tabled != 0): groups, per-subject group and loose-cell
+  // ranges, loose cells, base-panel index words, panel coefficients
+  const ZGroup* zgroup;
+  const uint32_t* zsg;
+  const uint32_t* zloose;
+  const uint32_t* zsl;
+  const uint32_t* zindex;
+  const double* zcoef;
+  uint32_t n_cells, nb, k, icpt, rw, tabled;
 };
 
 struct FitKnobs {
@@ -606,6 +615,167 @@ __device__ __forceinline__ void sweep_wave(const CT* __restrict__ rows,
   }
 }
 
+// One wave's part of a sweep on a design with zero-count tables, FP64
+// throughout (src/sc/pml.rs:1510-1560, :1869-1890): per subject of its block
+// [cb, ce), the zero-count sums of the subject's groups whose first cell lies
+// in the block (one table lookup each, lanes over groups), the loose cells in
+// the block, then what the positive counts in the block add -- every term is
+// linear in the count through gamma + y, so the split is exact.  Writes the
+// same per-subject slots (s + wave) and per-gene rows as sweep_wave.
+template <int MODE, int NB>
+__device__ __forceinline__ void sweep_wave_tabled(
+    const double* __restrict__ rows, const uint32_t* __restrict__ gcell,
+    const double* __restrict__ gy, const uint32_t* __restrict__ fid,
+    const ZGroup* __restrict__ zgroup, const uint32_t* __restrict__ zsg,
+    const uint32_t* __restrict__ zloose, const uint32_t* __restrict__ zsl,
+    const uint32_t* __restrict__ zindex, const double* __restrict__ zcoef,
+    const double* __restrict__ bsrc, const double* __restrict__ lwsrc, const lds_double* tab,
+    double* __restrict__ psub, double* __restrict__ pgene, uint64_t ip, uint64_t g1, int s, int k,
+    uint32_t cb, uint32_t ce, int icpt, double gamma, int lane, int wave) {
+  using D = Dims<NB>;
+  constexpr int Q = D::Q, R = D::R, NS = D::NS, NSL = D::NSL, NG = D::NG;
+  constexpr int NSV = MODE == SW_LAPLACE ? 4 : (MODE == SW_FINISH ? D::NS3 : NS);
+  constexpr int NGV = MODE == SW_LAPLACE ? 0 : NG;
+  constexpr int QA = Q > 0 ? Q : 1;
+  double bq[QA];
+#pragma unroll
+  for (int q = 0; q < QA; ++q) bq[q] = q < Q ? bsrc[q < icpt ? q : q + 1] : 0.0;
+  const double bint = bsrc[icpt];
+  const double lng = log(gamma);
+  double ag[NGV > 0 ? NGV : 1];
+#pragma unroll
+  for (int v = 0; v < (NGV > 0 ? NGV : 1); ++v) ag[v] = 0.0;
+  // adds one item: resid d, weight p, log-likelihood term tm, d p / d eta dp
+  // (SW_FINISH), or the four Laplace sums lp (SW_LAPLACE), at stored row xd
+  auto fold = [&](double* as, const double* xd, double d, double p, double tm, double dp,
+                  const double* lp) {
+    if constexpr (MODE == SW_LAPLACE) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) as[v] += lp[v];
+    } else {
+      as[0] += d;
+      as[1] += p;
+      ag[0] += tm;
+      if constexpr (MODE == SW_FINISH) {
+        as[NS] += dp;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) as[NS + 1 + q] += dp * xd[q];
+      }
+#pragma unroll
+      for (int a = 0; a < Q; ++a) {
+        const double pa = p * xd[a];
+        as[2 + a] += pa;
+        ag[1 + a] += xd[a] * d;
+#pragma unroll
+        for (int b = a; b < Q; ++b) ag[1 + Q + tri_index(Q, a, b)] += pa * xd[b];
+      }
+    }
+  };
+  // one cell: its zero-count terms (positive false) or what count y adds
+  auto cell = [&](double* as, uint32_t c, double y, bool positive, double lws) {
+    const double* rp = rows + (size_t)c * R;
+    double xd[QA];
+#pragma unroll
+    for (int q = 0; q < QA; ++q) xd[q] = q < Q ? rp[q] : 0.0;
+    double eta = rp[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) eta += xd[q] * bq[q];
+    eta += bint;
+    const double ex = fast_exp(eta + lws, tab);
+    const double t = ex + gamma;
+    const double inv = fast_rcp(t);
+    const double w = positive ? y : gamma;
+    if constexpr (MODE == SW_LAPLACE) {
+      const double p2 = ex * inv * inv, p3 = p2 * inv, p4 = p3 * inv, p5 = p4 * inv;
+      const double e2 = ex * ex;
+      double lp[4];
+      lp[0] = w * p2;
+      lp[1] = w * (p3 * (gamma - ex));
+      lp[2] = w * (p4 * (gamma * gamma + e2 - (4.0 * gamma) * ex));
+      lp[3] = w * (p5 * (gamma * gamma * gamma - (11.0 * gamma * gamma) * ex +
+                         (11.0 * gamma) * e2 - e2 * ex));
+      fold(as, xd, 0.0, 0.0, 0.0, 0.0, lp);
+    } else {
+      const double u = ex * inv;
+      const double lt = fast_log(t, tab);
+      const double p = w * u * inv;
+      const double d = positive ? y * (1.0 - u) : -gamma * u;
+      const double tm = positive ? y * (eta - lt) : -gamma * lt;
+      const double dp = MODE == SW_FINISH ? p * ((gamma - ex) * inv) : 0.0;
+      fold(as, xd, d, p, tm, dp, nullptr);
+    }
+  };
+  if (cb < ce) {
+    for (; s < k && fid[s] < ce; ++s) {
+      const uint32_t lo = max(fid[s], cb), hi = min(fid[s + 1], ce);
+      const double lws = lwsrc[s];
+      double as[NSV];
+#pragma unroll
+      for (int v = 0; v < NSV; ++v) as[v] = 0.0;
+      // groups whose first cell is in [lo, hi): v = exp(ln tau + ln O) with
+      // ln tau = log w + x'beta - ln gamma
+      for (uint32_t gi = zsg[s] + (uint32_t)lane; gi < zsg[s + 1]; gi += kWave) {
+        const ZGroup& G = zgroup[gi];
+        if (G.row < lo || G.row >= hi) continue;
+        const double* rp = rows + (size_t)G.row * R;
+        double xd[QA];
+#pragma unroll
+        for (int q = 0; q < QA; ++q) xd[q] = q < Q ? rp[q] : 0.0;
+        double st = lws - lng + bint;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) st += xd[q] * bq[q];
+        if constexpr (MODE == SW_LAPLACE) {
+          double f[4];
+          zt_eval<2, 4>(G, zindex, zcoef, st, f);  // gamma times each factor: B..E
+          fold(as, xd, 0.0, 0.0, 0.0, 0.0, f);
+        } else if constexpr (MODE == SW_FINISH) {
+          double f[4];
+          zt_eval<0, 4>(G, zindex, zcoef, st, f);
+          fold(as, xd, -gamma * f[1], f[2], -gamma * (G.n * lng + f[0]), f[3], nullptr);
+        } else {
+          double f[3];
+          zt_eval<0, 3>(G, zindex, zcoef, st, f);
+          fold(as, xd, -gamma * f[1], f[2], -gamma * (G.n * lng + f[0]), 0.0, nullptr);
+        }
+      }
+      // loose cells in [lo, hi): the first at or past lo, then in order
+      uint32_t a = zsl[s], b = zsl[s + 1];
+      const uint32_t l_end = b;
+      while (a < b) {
+        const uint32_t m = (a + b) / 2;
+        if (zloose[m] < lo) a = m + 1; else b = m;
+      }
+      for (uint32_t i = a + (uint32_t)lane; i < l_end; i += kWave) {
+        const uint32_t c = zloose[i];
+        if (c >= hi) break;
+        cell(as, c, 0.0, false, lws);
+      }
+      // positive counts with cells in [lo, hi): a prefix of the window at ip
+      for (;;) {
+        const uint64_t e = ip + (uint64_t)lane;
+        const uint32_t c = e < g1 ? gcell[e] : 0xffffffffu;
+        const bool in = c < hi;
+        if (in) cell(as, c, gy[e], true, lws);
+        const int m = popc64(wave_ballot(in));
+        ip += (uint64_t)m;
+        if (m < kWave) break;
+      }
+#pragma unroll
+      for (int v = 0; v < NSV; ++v) {
+        const double r = wave_sum(as[v]);
+        if (lane == 0) psub[(size_t)(s + wave) * NSL + v] = r;
+      }
+    }
+  }
+  if constexpr (NGV > 0) {
+#pragma unroll
+    for (int v = 0; v < NGV; ++v) {
+      const double r = wave_sum(ag[v]);
+      if (lane == 0) pgene[(size_t)wave * NG + v] = r;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // The fit of one request, all threads of the workgroup cooperating.
 template <int NB, int WAVES>
@@ -686,19 +856,34 @@ struct Fitter {
     __syncthreads();
   }
 
+  // This wave's part of a sweep on a design with zero-count tables, out of
+  // line so that the dense sweep's registers are allocated without it.
+  template <int MODE>
+  __device__ __noinline__ void wave_part_tabled(const double* bsrc, const double* lwsrc) {
+    sweep_wave_tabled<MODE, NB>(dd.row64, dd.gcell, dd.gy64, dd.fid, dd.zgroup, dd.zsg, dd.zloose,
+                                dd.zsl, dd.zindex, dd.zcoef, bsrc, lwsrc,
+                                (const lds_double*)&sh.f64tab[0], wk.psub, wk.pgene,
+                                sh.ip0[wave], g1, (int)sh.s0[wave], k, block_begin(wave),
+                                block_end(wave), icpt, gamma, lane, wave);
+  }
+
   // One pass over the gene's cells at (beta in LDS `bsrc`, log w `lwsrc`),
   // leaving per-subject totals in wk.tsub and per-gene totals in wk.tgene.
   template <class CT, int MODE>
   __device__ __noinline__ void sweep(const double* bsrc, const double* lwsrc) {
     constexpr int NSV = MODE == SW_LAPLACE ? 4 : (MODE == SW_FINISH ? D::NS3 : NS);
     constexpr int NGV = MODE == SW_LAPLACE ? 0 : NG;
-    const CT* rows = sizeof(CT) == 4 ? (const CT*)dd.row32 : (const CT*)dd.row64;
-    const CT* gy = sizeof(CT) == 4 ? (const CT*)dd.gy32 : (const CT*)dd.gy64;
-    sweep_wave<CT, MODE, NB>(rows, dd.gcell, gy, dd.fid, bsrc, lwsrc,
-                             (typename Lds<CT>::ptr)(&sh.ysc[wave][0]),
-                             (const lds_double*)&sh.f64tab[0], wk.psub, wk.pgene,
-                             sh.ip0[wave], g1, (int)sh.s0[wave], k, block_begin(wave),
-                             block_end(wave), icpt, gamma, lane, wave);
+    if (dd.tabled) {
+      wave_part_tabled<MODE>(bsrc, lwsrc);
+    } else {
+      const CT* rows = sizeof(CT) == 4 ? (const CT*)dd.row32 : (const CT*)dd.row64;
+      const CT* gy = sizeof(CT) == 4 ? (const CT*)dd.gy32 : (const CT*)dd.gy64;
+      sweep_wave<CT, MODE, NB>(rows, dd.gcell, gy, dd.fid, bsrc, lwsrc,
+                               (typename Lds<CT>::ptr)(&sh.ysc[wave][0]),
+                               (const lds_double*)&sh.f64tab[0], wk.psub, wk.pgene,
+                               sh.ip0[wave], g1, (int)sh.s0[wave], k, block_begin(wave),
+                               block_end(wave), icpt, gamma, lane, wave);
+    }
     __syncthreads();
     // per-subject totals, waves added in order; per-gene totals likewise
     for (int idx = tid; idx < k * NSV; idx += NT) {

@@ -123,7 +123,28 @@ DeviceData Resident::view() const {
   d.k = (uint32_t)k;
   d.icpt = (uint32_t)icpt;
   d.rw = rw;
+  d.tabled = tabled ? 1u : 0u;
+  d.zgroup = zgroup.p;
+  d.zsg = zsg.p;
+  d.zloose = zloose.p;
+  d.zsl = zsl.p;
+  d.zindex = zindex.p;
+  d.zcoef = zcoef.p;
   return d;
+}
+
+void Resident::set_tables(const ZeroTables& z) {
+  if (z.subject_groups.size() != k + 1 || z.subject_loose.size() != k + 1)
+    throw std::invalid_argument("zero-count tables built for another subject count");
+  NH_HIP(hipSetDevice(device));
+  zgroup.upload(z.groups.data(), z.groups.size(), stream);
+  zsg.upload(z.subject_groups.data(), z.subject_groups.size(), stream);
+  zloose.upload(z.loose.data(), z.loose.size(), stream);
+  zsl.upload(z.subject_loose.data(), z.subject_loose.size(), stream);
+  zindex.upload(z.index.data(), z.index.size(), stream);
+  zcoef.upload(z.coef.data(), z.coef.size(), stream);
+  NH_HIP(hipStreamSynchronize(stream));
+  tabled = true;
 }
 
 void Resident::reset_counters() {

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "zero_tables.hpp"
 
 namespace nh {
 
@@ -95,6 +96,11 @@ struct Resident {
   DevBuf<uint32_t> fid, gcell;
   DevBuf<uint64_t> gptr;
   DevBuf<FitCounters> counters;
+  // zero-count tables, when set_tables() has put them in place
+  bool tabled = false;
+  DevBuf<ZGroup> zgroup;
+  DevBuf<uint32_t> zsg, zloose, zsl, zindex;
+  DevBuf<double> zcoef;
   // per-solve staging
   DevBuf<DevRequest> req;
   DevBuf<DevReply> reply;
@@ -115,6 +121,9 @@ struct Resident {
               const uint64_t* subject_start, size_t n_genes, const uint64_t* gene_ptr,
               const uint32_t* cells, const double* counts, int device);
   DeviceData view() const;
+  // Uploads a run's zero-count tables; every later fit's sweeps read them
+  // (one lookup per group, the loose cells and the positive counts).
+  void set_tables(const ZeroTables& z);
   // Runs n requests, replies filled on the host (submit + collect).
   // decrement_stop: mode 0 fits stop on the Newton decrement (stage-two
   // evaluations).

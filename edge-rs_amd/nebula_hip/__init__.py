@@ -92,7 +92,8 @@ class _Options(C.Structure):
     _fields_ = [("engine", C.c_int32), ("learned", C.c_int32),
                 ("mlp_weights", _fp), ("wg_size", C.c_int32),
                 ("device", C.c_int32), ("host_threads", C.c_int32),
-                ("learned_step", C.c_double), ("learned_xtol", C.c_double)]
+                ("learned_step", C.c_double), ("learned_xtol", C.c_double),
+                ("zero_tables", C.c_int32)]
 
 
 class RunStats(C.Structure):
@@ -106,7 +107,8 @@ class RunStats(C.Structure):
                 ("nnz_sweeps", C.c_uint64), ("warm_fallbacks", C.c_uint64),
                 ("fit_kernel_ms", C.c_double), ("fit_kernel_launches", C.c_uint64),
                 ("cell_sweeps_f64", C.c_uint64), ("nnz_sweeps_f64", C.c_uint64),
-                ("t_plan", C.c_double)]
+                ("t_plan", C.c_double), ("zero_groups", C.c_uint64),
+                ("t_tables", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -255,7 +257,7 @@ def nebula_sparse_hip(counts: CompressedSparse, subject_id, design, n_coef: int,
                       engine: int = ENGINE_BATCHED, learned: bool = False,
                       mlp_weights=None, wg_size: int = 0, device: int = 0,
                       host_threads: int = 0, learned_step: float = 0.0,
-                      learned_xtol: float = 0.0) -> NebulaFit:
+                      learned_xtol: float = 0.0, zero_tables: bool = True) -> NebulaFit:
     """NEBULA with stages one to three on the GPU (edge-rs nebula_sparse_gpu)."""
     if counts.format != "csr":
         raise MalformedSparse("nebula_sparse needs gene-major counts in CSR form.")
@@ -289,6 +291,7 @@ def nebula_sparse_hip(counts: CompressedSparse, subject_id, design, n_coef: int,
     o.host_threads = host_threads
     o.learned_step = learned_step
     o.learned_xtol = learned_xtol
+    o.zero_tables = int(zero_tables)
     fit = _Fit()
     st = RunStats()
     err = C.create_string_buffer(1024)
@@ -500,6 +503,45 @@ def selftest_copy(mib=256, reps=1, device=0):
         _raise(rc, err)
 
 
+def zero_tables_probe(design, log_offset, subject_start, s=(), frm=0, m=3):
+    """The library's zero-count tables built on the host (nh_zero_tables_probe):
+    None when they do not apply, else a dict of group_row, subject_groups and
+    `values`, the sums from `frm` (m of them) at each point of s, shaped
+    (n_groups, len(s), m)."""
+    design = np.ascontiguousarray(design, dtype=np.float64)
+    lo = np.ascontiguousarray(log_offset, dtype=np.float64)
+    n = len(lo)
+    nb = design.size // n
+    ss = np.ascontiguousarray(subject_start, dtype=np.uint64)
+    k = len(ss) - 1
+    pts = np.ascontiguousarray(np.atleast_1d(np.asarray(s, dtype=np.float64)))
+    cap = n * max(1, len(pts)) * m
+    out = np.zeros(cap)
+    applies = C.c_int32(0)
+    n_groups = C.c_uint64(0)
+    rows = np.zeros(n, dtype=np.uint64)
+    sg = np.zeros(k + 1, dtype=np.uint64)
+    L = lib()
+    L.nh_zero_tables_probe.restype = C.c_int
+    L.nh_zero_tables_probe.argtypes = [_dp, _dp, C.c_size_t, C.c_size_t, _u64p, C.c_size_t, _dp,
+                                       C.c_size_t, C.c_int32, C.c_int32, _dp, C.c_size_t,
+                                       C.POINTER(C.c_int32), C.POINTER(C.c_uint64), _u64p, _u64p,
+                                       C.c_char_p, C.c_size_t]
+    err = C.create_string_buffer(512)
+    rc = L.nh_zero_tables_probe(design.ctypes.data_as(_dp), lo.ctypes.data_as(_dp), n, nb,
+                                ss.ctypes.data_as(_u64p), k, pts.ctypes.data_as(_dp), len(pts),
+                                frm, m, out.ctypes.data_as(_dp), cap, C.byref(applies),
+                                C.byref(n_groups), rows.ctypes.data_as(_u64p),
+                                sg.ctypes.data_as(_u64p), err, 512)
+    if rc != 0:
+        _raise(rc, err)
+    if not applies.value:
+        return None
+    g = n_groups.value
+    return dict(group_row=rows[:g].astype(np.int64), subject_groups=sg.astype(np.int64),
+                values=out[: g * len(pts) * m].reshape(g, len(pts), m))
+
+
 VALUE_GRAD_FN = C.CFUNCTYPE(C.c_double, _dp, _dp, C.c_void_p)
 
 
@@ -666,7 +708,7 @@ class Session:
 
     def __init__(self, counts: CompressedSparse, subject_id, design, n_coef, offset=None,
                  params: Optional[NebulaParams] = None, wg_size=0, device=0, host_threads=0,
-                 mlp_weights=None, learned_step=0.0, learned_xtol=0.0):
+                 mlp_weights=None, learned_step=0.0, learned_xtol=0.0, zero_tables=True):
         L = lib()
         if not hasattr(L, "_session_bound"):
             L.nh_session_create.restype = C.c_int
@@ -717,6 +759,7 @@ class Session:
         o.host_threads = host_threads
         o.learned_step = learned_step
         o.learned_xtol = learned_xtol
+        o.zero_tables = int(zero_tables)
         w = None
         if mlp_weights is not None:
             w = np.ascontiguousarray(mlp_weights, dtype=np.float32)

@@ -79,6 +79,15 @@ def generate(name, n_genes, n_cells, k, p, sizes="equal", beta0=("normal", -3.0,
         batch = rng.integers(0, 4, size=n_cells)
         for level in (1, 2, 3):
             cols.append((batch == level).astype(float))
+    elif design_kind == "categorical":
+        # a three-level cell type (two indicator columns) and a rare flag
+        # (~2 % of cells): every column but the flag is shared by large groups
+        # of a subject's cells, so the zero-count tables apply and the flagged
+        # cells stay loose
+        ctype = rng.integers(0, 3, size=n_cells)
+        cols.append((ctype == 1).astype(float))
+        cols.append((ctype == 2).astype(float))
+        cols.append((rng.random(n_cells) < 0.02).astype(float))
     elif design_kind == "generic":
         while len(cols) < p:
             cols.append(rng.normal(size=n_cells))

@@ -115,6 +115,9 @@ typedef struct {
   double learned_step;   /* warm search's initial BOBYQA step relative to the
                             predicted point (0 = default 0.01) */
   double learned_xtol;   /* warm search's rhoend / rhobeg (0 = default 3e-2) */
+  int32_t zero_tables;   /* 1 (default): where zero-count tables apply
+                            (src/sc/zeros.rs:219-262) every sweep reads them;
+                            0: sweep every cell */
 } nebula_hip_options_t;
 
 /* Per-phase timings and counters of one call (EDGE_RS_GPU_TIMING's phases,
@@ -137,6 +140,8 @@ typedef struct {
   double t_plan;             /* seconds of t_stage_two spent building the run's plan
                                 (search descriptors, count tails, features, buffers,
                                 the captured launch); 0 when a session replays one */
+  uint64_t zero_groups;      /* zero-count table groups the sweeps read (0: none) */
+  double t_tables;           /* seconds of t_setup spent building the tables */
 } nebula_run_stats_t;
 
 void nebula_params_default(nebula_params_t* params);
@@ -381,6 +386,23 @@ int nh_stage_two_search_device(int problem, int kind, int ord, const double* sta
  * yardstick the HBM byte counters (FETCH_SIZE, WRITE_SIZE) are scaled by.
  * No counterpart in edge-rs.  Returns NH_OK or an NH_ERR_* code. */
 int nh_selftest_copy(size_t mib, int32_t reps, int32_t device, char* err, size_t err_len);
+
+/* The zero-count tables the device sweeps read (src/sc/zeros.rs:219-278's
+ * ZeroCells::build and :141-193's Group::eval), built and evaluated on the
+ * host: for a centred design (n_cells x n_coef), log offsets and subject
+ * starts (n_subjects + 1), *applies is 1 when the tables apply and 0 when the
+ * cells in groups of >= 32 identical rows are under half the run.  When they
+ * apply: *n_groups groups; group_row (capacity n_cells) their first cells;
+ * subject_groups (n_subjects + 1); and, for each group g and point i,
+ * out[(g * n_s + i) * m + j] = sum from + j at s[i] (from, m one of (0, 2),
+ * (0, 3), (0, 4), (2, 4), (0, 6)); out_len bounds out.  A test hook: no
+ * counterpart in edge-rs's API.  Returns NH_OK or an NH_ERR_* code. */
+int nh_zero_tables_probe(const double* design, const double* log_offset, size_t n_cells,
+                         size_t n_coef, const uint64_t* subject_start, size_t n_subjects,
+                         const double* s, size_t n_s, int32_t from, int32_t m, double* out,
+                         size_t out_len, int32_t* applies, uint64_t* n_groups,
+                         uint64_t* group_row, uint64_t* subject_groups, char* err,
+                         size_t err_len);
 
 /* The FP64 sweep's exp (which 0), log (1) and reciprocal (2), and the device
  * objective's log Gamma for x > 0 (3), applied on the device to x[0..n), for

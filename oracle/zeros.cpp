@@ -180,6 +180,11 @@ ZeroGroup build_zero_group(std::size_t row, const std::vector<double>& log_o) {
 }
 
 void ZeroGroup::eval(int from, int m, double s, double* out) const {
+  if (std::isnan(s)) {
+    // every sum is NaN (the reference's saturating casts reach the same end)
+    for (int j = 0; j < m; ++j) out[j] = s;
+    return;
+  }
   if (s < s_lo) {
     const double v = std::exp(s) * sum_o;
     for (int j = 0; j < m; ++j) out[j] = v;

@@ -1,0 +1,115 @@
+"""The device sweeps on zero-count tables (src/sc/zeros.rs; the sweep split of
+src/sc/pml.rs:1510-1560 and the Laplace terms of :1869-1890): on a design
+whose cells fall into large groups sharing a design row, every engine's
+minimisers re-evaluated by the oracle's f64 profile objective are within
+|dloglik| < 1e-6 of the oracle's (which reads its own tables), the objective
+the device reports at its minimiser matches the oracle's there (the tabled
+FP64 finish and the tabled Laplace terms, not just the search), and the run
+that sweeps every cell reaches the same fits."""
+import numpy as np
+import pytest
+
+import oracle
+import nebula_hip as nh
+from fixtures import assert_close
+from nebula_hip import synth
+
+pytestmark = pytest.mark.gpu
+
+ENGINES = {"plain": nh.ENGINE_PLAIN, "batched": nh.ENGINE_BATCHED,
+           "learned": nh.ENGINE_LEARNED}
+_cache = {}
+
+
+def _workload():
+    if "w" not in _cache:
+        # 30 subjects x 400 cells; intercept, subject-level treatment, a
+        # three-level cell type and a rare flag (loose cells); the least
+        # expressed genes take the order-3 Laplace path
+        w = synth.generate("categorical", 24, 12_000, 30, 5, "equal", ("uniform", -6.5, 0.5),
+                           seed=31, design_kind="categorical")
+        prm = oracle.params(method="HL")
+        ref = oracle.nebula_sparse(w.indptr, w.indices, w.data, w.n_cells, w.subject_id,
+                                   w.design, w.n_coef, w.offset, prm, with_stage_two=True)
+        _cache["w"] = (w, prm, ref)
+    return _cache["w"]
+
+
+def _run(w, engine, zero_tables=True):
+    counts = nh.CompressedSparse(w.data, w.indices, w.indptr, (w.n_genes, w.n_cells))
+    return nh.nebula_sparse_hip(counts, w.subject_id, w.design, w.n_coef, w.offset,
+                                nh.NebulaParams(method="HL"), engine=ENGINES[engine],
+                                learned=engine == "learned",
+                                mlp_weights=nh.load_weights() if engine == "learned" else None,
+                                zero_tables=zero_tables)
+
+
+def test_the_tables_apply_with_loose_cells():
+    w, _, ref = _workload()
+    fid = np.r_[0, np.cumsum(w.truth["sizes"])]
+    z = oracle.zero_cells(w.design, np.log(w.offset), fid)
+    assert z is not None and len(z["loose"]) > 0
+    # both Laplace orders are searched
+    assert set(ref["stage_two"][ref["stage_two"][:, 0] > 0, 5].astype(int)) == {1, 3}
+
+
+@pytest.mark.parametrize("engine", list(ENGINES))
+def test_tabled_stage_two_matches_oracle(engine):
+    w, prm, ref = _workload()
+    fit = _run(w, engine)
+    print(f"tables {engine}: {fit.stats['zero_groups']} groups, "
+          f"evaluations {fit.stats['evaluations']}", flush=True)
+    assert fit.stats["zero_groups"] > 0
+    assert np.array_equal(fit.gene_index, ref["gene_index"])
+    s2 = ref["stage_two"]
+    g = np.nonzero(s2[:, 0] > 0)[0]
+    pts = np.c_[fit.subject_overdispersion, 1.0 / fit.cell_overdispersion][g]
+    args = (w.indptr, w.indices, w.data, w.n_cells, w.subject_id, w.design, w.n_coef, w.offset,
+            prm, g)
+    v_hip = oracle.profile_objective(*args, pts)
+    v_cpu = oracle.profile_objective(*args, s2[g, 1:3])
+    rel = np.abs(v_hip - v_cpu) / np.abs(v_cpu)
+    assert np.all(np.isfinite(rel)) and rel.max() < 1e-6, (rel.max(), int(g[np.argmax(rel)]))
+    assert_close(fit.coefficients, ref["coefficients"], 1e-2, 1e-4, label="coef")
+    assert_close(fit.se, ref["se"], 1e-2, label="se")
+    assert_close(fit.subject_overdispersion, ref["subject_overdispersion"], 1.5e-2, 1e-3,
+                 label="subject")
+    assert_close(fit.cell_overdispersion, ref["cell_overdispersion"], 1e-2, label="cell")
+
+
+@pytest.mark.parametrize("engine", ["plain", "batched"])
+def test_tabled_objective_matches_oracle_at_the_minimiser(engine):
+    """The objective the device search reports at its minimiser (tabled FP64
+    finish, tabled Laplace order 3) against the oracle's at the same point."""
+    w, prm, ref = _workload()
+    counts = nh.CompressedSparse(w.data, w.indices, w.indptr, (w.n_genes, w.n_cells))
+    s = nh.Session(counts, w.subject_id, w.design, w.n_coef, w.offset,
+                   nh.NebulaParams(method="HL"))
+    try:
+        r = s.stage_two(ENGINES[engine])
+    finally:
+        s.close()
+    g = np.arange(len(r["minimisers"]))
+    v = oracle.profile_objective(w.indptr, w.indices, w.data, w.n_cells, w.subject_id, w.design,
+                                 w.n_coef, w.offset, prm, g, r["minimisers"])
+    rel = np.abs(r["objectives"] - v) / np.abs(v)
+    assert np.all(np.isfinite(rel))
+    assert np.median(rel) < 1e-8 and rel.max() < 1e-5, (np.median(rel), rel.max())
+
+
+def test_tabled_and_dense_device_runs_agree():
+    """The same engine with the tables and sweeping every cell: both reach
+    minimisers within |dloglik| < 1e-6 of each other."""
+    w, prm, ref = _workload()
+    a = _run(w, "batched", zero_tables=True)
+    b = _run(w, "batched", zero_tables=False)
+    assert a.stats["zero_groups"] > 0 and b.stats["zero_groups"] == 0
+    g = np.nonzero(ref["stage_two"][:, 0] > 0)[0]
+    args = (w.indptr, w.indices, w.data, w.n_cells, w.subject_id, w.design, w.n_coef, w.offset,
+            prm, g)
+    va = oracle.profile_objective(*args, np.c_[a.subject_overdispersion,
+                                               1.0 / a.cell_overdispersion][g])
+    vb = oracle.profile_objective(*args, np.c_[b.subject_overdispersion,
+                                               1.0 / b.cell_overdispersion][g])
+    assert np.max(np.abs(va - vb) / np.abs(vb)) < 1e-6
+    assert_close(a.coefficients, b.coefficients, 1e-2, 1e-4, label="coef")

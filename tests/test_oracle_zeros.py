@@ -196,3 +196,39 @@ def test_tabled_and_dense_runs_agree_end_to_end():
     assert rel("se") < 5e-5
     assert rel("cell_overdispersion") < 5e-5
     assert rel("subject_overdispersion") < 2e-4
+
+
+def test_library_tables_match_the_oracle():
+    """The library's host-built tables (the ones the device sweeps read): the
+    same groups as the oracle's ZeroCells::build, and every sum of every group
+    within 1e-13 of the oracle's compensated direct sums over the group's
+    offsets, across the whole table and both tails."""
+    import nebula_hip as nh
+    n = 600
+    rng = np.random.default_rng(21)
+    log_o = rng.uniform(-1.5, 2.5, n)
+    design = np.array([[1.0, r % 3, 1.0 if r % 97 == 0 else 0.0, (r // 300) * 0.5]
+                       for r in range(n)])
+    fid = [0, 300, 600]
+    want = oracle.zero_cells(design, log_o, fid)
+    got_grouping = nh.zero_tables_probe(design, log_o, fid, [0.0])
+    np.testing.assert_array_equal(got_grouping["group_row"], want["group_row"])
+    np.testing.assert_array_equal(got_grouping["subject_groups"], want["subject_groups"])
+    members = {}
+    for r in range(n):
+        key = tuple(design[r])
+        s = 0 if r < 300 else 1
+        members.setdefault((s, key), []).append(r)
+    s_pts = np.arange(-50.0, 50.0, 0.173)
+    got = nh.zero_tables_probe(design, log_o, fid, s_pts, frm=0, m=6)["values"]
+    for g, row in enumerate(want["group_row"]):
+        s = 0 if row < 300 else 1
+        cells = members[(s, tuple(design[row]))]
+        _, direct, _ = oracle.zero_group(log_o[cells], s_pts)
+        err = _scaled_error(got[g], direct)
+        assert err.max() < 1e-13, (g, err.max(axis=0))
+    # (2, 4): B..E, as the Laplace terms read them
+    b_e = nh.zero_tables_probe(design, log_o, fid, s_pts, frm=2, m=4)["values"]
+    np.testing.assert_allclose(b_e, got[:, :, 2:6], rtol=1e-15, atol=0)
+    x = np.random.default_rng(9).uniform(-1, 1, n)
+    assert nh.zero_tables_probe(np.column_stack([np.ones(n), x]), log_o, fid) is None
