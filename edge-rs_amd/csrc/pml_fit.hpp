@@ -108,6 +108,11 @@ enum SweepMode { SW_FULL = 0, SW_FINISH = 1, SW_LAPLACE = 2 };
 #endif
 constexpr int kTileF32 = NH_TILE_F32;
 constexpr int kTileF64 = NH_TILE_F64;
+// floats in one wave's count staging row: a tile's counts in either type
+constexpr int kStageFloats =
+    kWave * (kTileF32 > 2 * kTileF64 ? kTileF32 : 2 * kTileF64) < kWave * 4
+        ? kWave * 4
+        : kWave * (kTileF32 > 2 * kTileF64 ? kTileF32 : 2 * kTileF64);
 
 template <int NB>
 struct Dims {
@@ -192,7 +197,7 @@ struct FitLds {
   double f64tab[82];  // tables of the FP64 sweep's exp and log (Fitter::setup)
   uint64_t ip0[WAVES];
   uint32_t s0[WAVES];
-  float ysc[WAVES][kWave * 4];  // count staging: 256 floats or 128 doubles per wave
+  float ysc[WAVES][kStageFloats];  // count staging: one tile's counts per wave
 };
 
 // What one fit hands back.
@@ -452,6 +457,64 @@ __device__ __forceinline__ void fill_f64_tables(double* tab, int tid) {
   }
 }
 
+// One cell's terms at count y added to the per-subject sums `as` and the
+// per-gene sums `ag`: eta = log offset + x'beta, e = exp(eta + log w) (c0 =
+// intercept + log w), t = e + gamma; the sweep modes' sums as listed above.
+template <class CT, int MODE, int NB>
+__device__ __forceinline__ void add_cell(const CT* xr, CT y, const CT* bq, CT bint, CT c0, CT g,
+                                         double gamma, const lds_double* tab, double* as,
+                                         double* ag) {
+  using D = Dims<NB>;
+  constexpr int Q = D::Q, NS = D::NS;
+  constexpr int QA = Q > 0 ? Q : 1;
+  CT eta = xr[Q];  // log offset
+#pragma unroll
+  for (int q = 0; q < Q; ++q) eta += xr[q] * bq[q];
+  const CT ex = fast_exp(eta + c0, tab);
+  eta += bint;
+  const CT t = ex + g;
+  const CT inv = fast_rcp(t);
+  if constexpr (MODE == SW_LAPLACE) {
+    // (gamma + y) times the four derivative factors (pml.rs:1861-1874)
+    double e64 = (double)ex, t64 = e64 + gamma, gs64 = gamma + (double)y;
+    double p2 = e64 / (t64 * t64), p3 = p2 / t64, p4 = p3 / t64, p5 = p4 / t64;
+    double e2 = e64 * e64;
+    as[0] += gs64 * p2;
+    as[1] += gs64 * (p3 * (gamma - e64));
+    as[2] += gs64 * (p4 * (gamma * gamma + e2 - (4.0 * gamma) * e64));
+    as[3] += gs64 * (p5 * (gamma * gamma * gamma - (11.0 * gamma * gamma) * e64 +
+                           (11.0 * gamma) * e2 - e2 * e64));
+  } else {
+    const CT u_ = ex * inv;
+    const CT gs = g + y;
+    const CT lt = fast_log(t, tab);
+    const double pw = (double)(gs * u_ * inv);
+    const double dres = (double)y - (double)gs * (double)u_;
+    const double term = (double)(y * eta) - (double)gs * (double)lt;
+    as[0] += dres;
+    as[1] += pw;
+    ag[0] += term;
+    double xd[QA];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) xd[q] = (double)xr[q];
+    if constexpr (MODE == SW_FINISH) {
+      // d pw / d eta = pw (gamma - e) / t
+      const double dpw = pw * (((double)g - (double)ex) * (double)inv);
+      as[NS] += dpw;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) as[NS + 1 + q] += dpw * xd[q];
+    }
+#pragma unroll
+    for (int a = 0; a < Q; ++a) {
+      const double pa = pw * xd[a];
+      as[2 + a] += pa;
+      ag[1 + a] += xd[a] * dres;
+#pragma unroll
+      for (int b = a; b < Q; ++b) ag[1 + Q + tri_index(Q, a, b)] += pa * xd[b];
+    }
+  }
+}
+
 // One wave's part of a sweep: cells [cb, ce), subjects from s, CSR cursor ip
 // (first entry at or past cb).  Writes the wave's per-subject sums to psub
 // (slot s + wave) and its per-gene sums to pgene (row wave).  Everything it
@@ -549,53 +612,7 @@ __device__ __forceinline__ void sweep_wave(const CT* __restrict__ rows,
         for (int u = 0; u < U; ++u) {
           const uint32_t cell = base + (uint32_t)(lane + kWave * u);
           if (cell >= tend) continue;
-          const CT y = yv[u];
-          CT eta = xr[u][Q];  // log offset
-#pragma unroll
-          for (int q = 0; q < Q; ++q) eta += xr[u][q] * bq[q];
-          const CT ex = fast_exp(eta + c0, tab);
-          eta += bint;
-          const CT t = ex + g;
-          const CT inv = fast_rcp(t);
-          if constexpr (MODE == SW_LAPLACE) {
-            // (gamma + y) times the four derivative factors (pml.rs:1861-1874)
-            double e64 = (double)ex, t64 = e64 + gamma, gs64 = gamma + (double)y;
-            double p2 = e64 / (t64 * t64), p3 = p2 / t64, p4 = p3 / t64, p5 = p4 / t64;
-            double e2 = e64 * e64;
-            as[0] += gs64 * p2;
-            as[1] += gs64 * (p3 * (gamma - e64));
-            as[2] += gs64 * (p4 * (gamma * gamma + e2 - (4.0 * gamma) * e64));
-            as[3] += gs64 * (p5 * (gamma * gamma * gamma - (11.0 * gamma * gamma) * e64 +
-                                   (11.0 * gamma) * e2 - e2 * e64));
-          } else {
-            const CT u_ = ex * inv;
-            const CT gs = g + y;
-            const CT lt = fast_log(t, tab);
-            const double pw = (double)(gs * u_ * inv);
-            const double dres = (double)y - (double)gs * (double)u_;
-            const double term = (double)(y * eta) - (double)gs * (double)lt;
-            as[0] += dres;
-            as[1] += pw;
-            ag[0] += term;
-            double xd[QA];
-#pragma unroll
-            for (int q = 0; q < Q; ++q) xd[q] = (double)xr[u][q];
-            if constexpr (MODE == SW_FINISH) {
-              // d pw / d eta = pw (gamma - e) / t
-              const double dpw = pw * (((double)g - (double)ex) * (double)inv);
-              as[NS] += dpw;
-#pragma unroll
-              for (int q = 0; q < Q; ++q) as[NS + 1 + q] += dpw * xd[q];
-            }
-#pragma unroll
-            for (int a = 0; a < Q; ++a) {
-              const double pa = pw * xd[a];
-              as[2 + a] += pa;
-              ag[1 + a] += xd[a] * dres;
-#pragma unroll
-              for (int b = a; b < Q; ++b) ag[1 + Q + tri_index(Q, a, b)] += pa * xd[b];
-            }
-          }
+          add_cell<CT, MODE, NB>(xr[u], yv[u], bq, bint, c0, g, gamma, tab, as, ag);
         }
       }
       // flush subject s: wave sum, slot s + wave
@@ -847,7 +864,7 @@ struct Fitter {
       if (dd.fid[m] <= cb) a = m; else b = m;
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) sh.ysc[wave][lane + kWave * u] = 0.0f;
+    for (int u = 0; u < kStageFloats / kWave; ++u) sh.ysc[wave][lane + kWave * u] = 0.0f;
     fill_f64_tables(sh.f64tab, tid);
     if (lane == 0) {
       sh.ip0[wave] = ip;
