@@ -257,7 +257,7 @@ def nebula_sparse_hip(counts: CompressedSparse, subject_id, design, n_coef: int,
                       engine: int = ENGINE_BATCHED, learned: bool = False,
                       mlp_weights=None, wg_size: int = 0, device: int = 0,
                       host_threads: int = 0, learned_step: float = 0.0,
-                      learned_xtol: float = 0.0, zero_tables: bool = True) -> NebulaFit:
+                      learned_xtol: float = 0.0, zero_tables: int = 1) -> NebulaFit:
     """NEBULA with stages one to three on the GPU (edge-rs nebula_sparse_gpu)."""
     if counts.format != "csr":
         raise MalformedSparse("nebula_sparse needs gene-major counts in CSR form.")
@@ -708,7 +708,7 @@ class Session:
 
     def __init__(self, counts: CompressedSparse, subject_id, design, n_coef, offset=None,
                  params: Optional[NebulaParams] = None, wg_size=0, device=0, host_threads=0,
-                 mlp_weights=None, learned_step=0.0, learned_xtol=0.0, zero_tables=True):
+                 mlp_weights=None, learned_step=0.0, learned_xtol=0.0, zero_tables=1):
         L = lib()
         if not hasattr(L, "_session_bound"):
             L.nh_session_create.restype = C.c_int

@@ -115,9 +115,11 @@ typedef struct {
   double learned_step;   /* warm search's initial BOBYQA step relative to the
                             predicted point (0 = default 0.01) */
   double learned_xtol;   /* warm search's rhoend / rhobeg (0 = default 3e-2) */
-  int32_t zero_tables;   /* 1 (default): where zero-count tables apply
-                            (src/sc/zeros.rs:219-262) every sweep reads them;
-                            0: sweep every cell */
+  int32_t zero_tables;   /* where zero-count tables apply (src/sc/zeros.rs:219-262):
+                            1 (default) the sweeps read them when their groups
+                            average at least 4,096 cells (below that a table
+                            lookup's dependent loads cost more on the device
+                            than sweeping the group's cells); 2 always; 0 never */
 } nebula_hip_options_t;
 
 /* Per-phase timings and counters of one call (EDGE_RS_GPU_TIMING's phases,

@@ -5,7 +5,9 @@ minimisers re-evaluated by the oracle's f64 profile objective are within
 |dloglik| < 1e-6 of the oracle's (which reads its own tables), the objective
 the device reports at its minimiser matches the oracle's there (the tabled
 FP64 finish and the tabled Laplace terms, not just the search), and the run
-that sweeps every cell reaches the same fits."""
+that sweeps every cell reaches the same fits.  The groups here are small
+(~130 cells), where the library's default sweeps every cell; the tests force
+the tables (zero_tables = 2)."""
 import numpy as np
 import pytest
 
@@ -35,7 +37,7 @@ def _workload():
     return _cache["w"]
 
 
-def _run(w, engine, zero_tables=True):
+def _run(w, engine, zero_tables=2):
     counts = nh.CompressedSparse(w.data, w.indices, w.indptr, (w.n_genes, w.n_cells))
     return nh.nebula_sparse_hip(counts, w.subject_id, w.design, w.n_coef, w.offset,
                                 nh.NebulaParams(method="HL"), engine=ENGINES[engine],
@@ -92,7 +94,7 @@ def test_tabled_objective_matches_oracle_at_the_minimiser(engine):
     w, prm, ref = _workload()
     counts = nh.CompressedSparse(w.data, w.indices, w.indptr, (w.n_genes, w.n_cells))
     s = nh.Session(counts, w.subject_id, w.design, w.n_coef, w.offset,
-                   nh.NebulaParams(method="HL"))
+                   nh.NebulaParams(method="HL"), zero_tables=2)
     try:
         r = s.stage_two(ENGINES[engine])
     finally:
@@ -109,8 +111,8 @@ def test_tabled_and_dense_device_runs_agree():
     """The same engine with the tables and sweeping every cell: both reach
     minimisers within |dloglik| < 1e-6 of each other."""
     w, prm, ref = _workload()
-    a = _run(w, "batched", zero_tables=True)
-    b = _run(w, "batched", zero_tables=False)
+    a = _run(w, "batched", zero_tables=2)
+    b = _run(w, "batched", zero_tables=0)
     assert a.stats["zero_groups"] > 0 and b.stats["zero_groups"] == 0
     g = np.nonzero(ref["stage_two"][:, 0] > 0)[0]
     args = (w.indptr, w.indices, w.data, w.n_cells, w.subject_id, w.design, w.n_coef, w.offset,
@@ -121,3 +123,27 @@ def test_tabled_and_dense_device_runs_agree():
                                                1.0 / b.cell_overdispersion][g])
     assert np.max(np.abs(va - vb) / np.abs(vb)) < 1e-6
     assert_close(a.coefficients, b.coefficients, 1e-2, 1e-4, label="coef")
+
+
+def test_default_reads_tables_only_for_large_groups():
+    """The default (zero_tables = 1) sweeps every cell when the groups are
+    small (200 genes x 5,000 cells: ten groups of 500) and reads the tables
+    when they are large (50,000 cells: ten groups of 5,000), where its
+    minimisers match the oracle's to |dloglik| < 1e-6."""
+    prm = oracle.params(method="HL")
+    for cells, want_groups in ((5_000, 0), (50_000, 10)):
+        w = synth.config(1, n_genes=16, n_cells=cells)
+        counts = nh.CompressedSparse(w.data, w.indices, w.indptr, (w.n_genes, w.n_cells))
+        fit = nh.nebula_sparse_hip(counts, w.subject_id, w.design, w.n_coef, w.offset,
+                                   nh.NebulaParams(method="HL"))
+        assert fit.stats["zero_groups"] == want_groups, (cells, fit.stats["zero_groups"])
+        if want_groups:
+            ref = oracle.nebula_sparse(w.indptr, w.indices, w.data, w.n_cells, w.subject_id,
+                                       w.design, w.n_coef, w.offset, prm, with_stage_two=True)
+            g = np.nonzero(ref["stage_two"][:, 0] > 0)[0]
+            args = (w.indptr, w.indices, w.data, w.n_cells, w.subject_id, w.design, w.n_coef,
+                    w.offset, prm, g)
+            v_hip = oracle.profile_objective(*args, np.c_[fit.subject_overdispersion,
+                                                          1.0 / fit.cell_overdispersion][g])
+            v_cpu = oracle.profile_objective(*args, ref["stage_two"][g, 1:3])
+            assert np.max(np.abs(v_hip - v_cpu) / np.abs(v_cpu)) < 1e-6
