@@ -35,6 +35,7 @@ SOURCES = [
     "driver.cpp",
     "sc_test.cpp",
     "zero_tables.cpp",
+    "shrink.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

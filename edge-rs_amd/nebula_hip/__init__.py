@@ -683,6 +683,123 @@ def glm_sc_test(coefficients, covariance, coef=None, contrast=None):
     return out
 
 
+def sc_residual_df(n_cells, n_coef, n_subjects):
+    """n_cells - n_coef - (n_subjects - 1), at least 1 (nh_sc_residual_df,
+    sc_residual_df in src/sc/shrink.rs:90)."""
+    L = lib()
+    L.nh_sc_residual_df.restype = C.c_double
+    L.nh_sc_residual_df.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t]
+    return float(L.nh_sc_residual_df(int(n_cells), int(n_coef), int(n_subjects)))
+
+
+@dataclass
+class ShrinkParams:
+    """ShrinkParams (src/sc/shrink.rs:38-49): Winsorised (robust) prior fit."""
+    robust: bool = True
+
+
+@dataclass
+class ShrinkResult:
+    """ShrinkResult (src/sc/shrink.rs:52-71): one value per gene, df_prior one
+    per usable gene (robust fit) or one."""
+    phi_raw: np.ndarray
+    phi_post: np.ndarray
+    phi_prior: np.ndarray
+    df_residual: float
+    df_prior: np.ndarray
+    dispersion_shrunk: np.ndarray
+
+
+def shrink_sc_dispersion(dispersion, convergence, covariate=None, df_residual=None,
+                         params: Optional[ShrinkParams] = None) -> ShrinkResult:
+    """Empirical-Bayes shrinkage of the cell-level overdispersions
+    (nh_shrink_sc_dispersion, shrink_sc_dispersion in src/sc/shrink.rs:118):
+    phi = 1 / dispersion through limma's squeezeVar on df_residual degrees of
+    freedom, trended on `covariate` when given."""
+    params = params or ShrinkParams()
+    d = np.ascontiguousarray(dispersion, dtype=np.float64).ravel()
+    n = d.size
+    conv = np.ascontiguousarray(convergence, dtype=np.int32).ravel()
+    if conv.size != n:
+        raise LengthMismatch(f"convergence: expected {n}, got {conv.size}")
+    cov = None
+    if covariate is not None:
+        cov = np.ascontiguousarray(covariate, dtype=np.float64).ravel()
+        if cov.size != n:
+            raise LengthMismatch(f"covariate: expected {n}, got {cov.size}")
+    if df_residual is None:
+        raise InvalidArgument("shrink_sc_dispersion needs df_residual")
+    outs = [np.zeros(n) for _ in range(4)]
+    df_prior = np.zeros(max(n, 1))
+    n_df = C.c_size_t(0)
+    err = C.create_string_buffer(512)
+    L = lib()
+    L.nh_shrink_sc_dispersion.restype = C.c_int
+    L.nh_shrink_sc_dispersion.argtypes = [_dp, C.POINTER(C.c_int32), _dp, C.c_size_t,
+                                          C.c_double, C.c_int32, _dp, _dp, _dp, _dp, _dp,
+                                          C.POINTER(C.c_size_t), C.c_char_p, C.c_size_t]
+    rc = L.nh_shrink_sc_dispersion(
+        d.ctypes.data_as(_dp), conv.ctypes.data_as(C.POINTER(C.c_int32)),
+        cov.ctypes.data_as(_dp) if cov is not None else None, n, float(df_residual),
+        int(bool(params.robust)), *[o.ctypes.data_as(_dp) for o in outs],
+        df_prior.ctypes.data_as(_dp), C.byref(n_df), err, 512)
+    if rc != 0:
+        _raise(rc, err)
+    return ShrinkResult(phi_raw=outs[0], phi_post=outs[1], phi_prior=outs[2],
+                        df_residual=float(df_residual), df_prior=df_prior[:n_df.value].copy(),
+                        dispersion_shrunk=outs[3])
+
+
+def squeeze_var(var, df, covariate=None, robust=False):
+    """limma's squeezeVar for one shared df (nh_squeeze_var, squeeze_var in
+    src/limma/squeeze_var.rs:286, the legacy family).  Returns (var_post,
+    var_prior, df_prior), the priors of length 1 or n."""
+    v = np.ascontiguousarray(var, dtype=np.float64).ravel()
+    n = v.size
+    cov = None
+    if covariate is not None:
+        cov = np.ascontiguousarray(covariate, dtype=np.float64).ravel()
+        if cov.size != n:
+            raise LengthMismatch(f"covariate: expected {n}, got {cov.size}")
+    post = np.zeros(n)
+    prior = np.zeros(max(n, 1))
+    dfp = np.zeros(max(n, 1))
+    n_prior = C.c_size_t(0)
+    n_dfp = C.c_size_t(0)
+    err = C.create_string_buffer(512)
+    L = lib()
+    L.nh_squeeze_var.restype = C.c_int
+    L.nh_squeeze_var.argtypes = [_dp, C.c_size_t, C.c_double, _dp, C.c_int32, _dp, _dp,
+                                 C.POINTER(C.c_size_t), _dp, C.POINTER(C.c_size_t), C.c_char_p,
+                                 C.c_size_t]
+    rc = L.nh_squeeze_var(v.ctypes.data_as(_dp), n, float(df),
+                          cov.ctypes.data_as(_dp) if cov is not None else None,
+                          int(bool(robust)), post.ctypes.data_as(_dp), prior.ctypes.data_as(_dp),
+                          C.byref(n_prior), dfp.ctypes.data_as(_dp), C.byref(n_dfp), err, 512)
+    if rc != 0:
+        _raise(rc, err)
+    return post, prior[:n_prior.value].copy(), dfp[:n_dfp.value].copy()
+
+
+def shrink_selftest(which, x, y=None, a=0.0, b=0.0):
+    """The shrinkage's distribution functions on the host (nh_shrink_selftest):
+    which 0 F density, 1 F upper tail, 2 F quantile, 3 chi-square upper tail,
+    4 lowess(x, y, span a, b passes), 5 log x - digamma x, 6 trigamma,
+    7 inverse trigamma."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    yy = np.ascontiguousarray(y if y is not None else np.zeros_like(x), dtype=np.float64).ravel()
+    out = np.zeros_like(x)
+    L = lib()
+    L.nh_shrink_selftest.restype = C.c_int
+    L.nh_shrink_selftest.argtypes = [C.c_int32, _dp, _dp, C.c_size_t, C.c_double, C.c_double,
+                                     _dp]
+    rc = L.nh_shrink_selftest(which, x.ctypes.data_as(_dp), yy.ctypes.data_as(_dp), x.size,
+                              float(a), float(b), out.ctypes.data_as(_dp))
+    if rc != 0:
+        _raise(rc, C.create_string_buffer(b"shrink selftest failed", 64))
+    return out
+
+
 def selftest_f64_math(x, which, device=0):
     """The FP64 sweep's exp (which 0), log (1) or reciprocal (2) of x,
     evaluated on the device."""

@@ -435,6 +435,56 @@ int nh_glm_sc_test(const double* coefficients, size_t coefficients_len,
                    size_t err_len);
 
 /* ---------------------------------------------------------------------------
+ * Empirical-Bayes shrinkage of the cell-level overdispersions, host code on a
+ * fit's output (src/sc/shrink.rs:90-226).
+ * ------------------------------------------------------------------------- */
+
+/* sc_residual_df (src/sc/shrink.rs:90): n_cells - n_coef - (n_subjects - 1),
+ * at least 1. */
+double nh_sc_residual_df(size_t n_cells, size_t n_coef, size_t n_subjects);
+
+/* shrink_sc_dispersion (src/sc/shrink.rs:118): phi = 1 / dispersion (infinite
+ * at 0) of the genes with convergence == 1 and finite phi, floored at 1e-8,
+ * moderated by limma's squeezeVar (equal-df family) on df_residual degrees of
+ * freedom -- trended on `covariate` (n_genes values, NULL for none) when given,
+ * robust when `robust` is non-zero (ShrinkParams::robust, default on), each
+ * rung of the ladder trended -> untrended -> untrended non-robust reached only
+ * when the one above fails, unshrunk (df_prior {0}) when all fail or fewer
+ * than 3 genes are usable.  Writes n_genes values each to phi_raw, phi_post,
+ * phi_prior and dispersion_shrunk (1 / phi_post), genes outside the fit taking
+ * the median prior; df_prior (capacity n_genes) gets *n_df_prior values: one
+ * per usable gene from the robust fit, else 1.  Errors (ShrinkResult's):
+ * NH_ERR_INVALID_ARGUMENT for n_genes == 0 or df_residual not positive and
+ * finite. */
+int nh_shrink_sc_dispersion(const double* dispersion, const int32_t* convergence,
+                            const double* covariate, size_t n_genes, double df_residual,
+                            int32_t robust, double* phi_raw, double* phi_post,
+                            double* phi_prior, double* dispersion_shrunk, double* df_prior,
+                            size_t* n_df_prior, char* err, size_t err_len);
+
+/* limma's squeezeVar as squeeze_var runs it for one shared df
+ * (src/limma/squeeze_var.rs:286-391, the legacy family it picks then):
+ * fitFDist (:426), its spline trend on `covariate` (:501), or the Winsorised
+ * fit when `robust` is non-zero (:612, tail proportions 0.05 and 0.1, lowess
+ * span 0.4 when trended).  var: n finite non-negative values; covariate NULL
+ * or n values.  Writes var_post (n), var_prior (*n_var_prior values: 1 or n)
+ * and df_prior (*n_df_prior values: 1 or n); fewer than 3 genes come back
+ * unmoderated with df_prior {0}.  Returns NH_OK or NH_ERR_INVALID_ARGUMENT
+ * (with the failing step's message). */
+int nh_squeeze_var(const double* var, size_t n, double df, const double* covariate,
+                   int32_t robust, double* var_post, double* var_prior, size_t* n_var_prior,
+                   double* df_prior, size_t* n_df_prior, char* err, size_t err_len);
+
+/* The distribution functions the shrinkage is built on, applied to x[0..n)
+ * (a test hook, no counterpart in edge-rs's API): which 0 F(a, b) density,
+ * 1 F(a, b) upper tail, 2 F(a, b) quantile at lower-tail probability x, 3
+ * chi-square(a) upper tail, 4 stats::lowess(x, y, span a, b robustness
+ * passes), 5 log x - digamma x, 6 trigamma, 7 inverse trigamma; b may be
+ * infinite for 0-2.  Returns NH_OK or an NH_ERR_* code. */
+int nh_shrink_selftest(int32_t which, const double* x, const double* y, size_t n, double a,
+                       double b, double* out);
+
+/* ---------------------------------------------------------------------------
  * Warm-start MLP (no counterpart in edge-rs): 12 features -> 64 -> 64 -> 64
  * -> 2 (log sigma^2, log phi), GELU (tanh form).  Parameter blob layout:
  * feature mean[12], feature sd[12], W1[64x12], b1[64], W2[64x64], b2[64],
