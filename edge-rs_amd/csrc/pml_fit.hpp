@@ -457,64 +457,6 @@ __device__ __forceinline__ void fill_f64_tables(double* tab, int tid) {
   }
 }
 
-// One cell's terms at count y added to the per-subject sums `as` and the
-// per-gene sums `ag`: eta = log offset + x'beta, e = exp(eta + log w) (c0 =
-// intercept + log w), t = e + gamma; the sweep modes' sums as listed above.
-template <class CT, int MODE, int NB>
-__device__ __forceinline__ void add_cell(const CT* xr, CT y, const CT* bq, CT bint, CT c0, CT g,
-                                         double gamma, const lds_double* tab, double* as,
-                                         double* ag) {
-  using D = Dims<NB>;
-  constexpr int Q = D::Q, NS = D::NS;
-  constexpr int QA = Q > 0 ? Q : 1;
-  CT eta = xr[Q];  // log offset
-#pragma unroll
-  for (int q = 0; q < Q; ++q) eta += xr[q] * bq[q];
-  const CT ex = fast_exp(eta + c0, tab);
-  eta += bint;
-  const CT t = ex + g;
-  const CT inv = fast_rcp(t);
-  if constexpr (MODE == SW_LAPLACE) {
-    // (gamma + y) times the four derivative factors (pml.rs:1861-1874)
-    double e64 = (double)ex, t64 = e64 + gamma, gs64 = gamma + (double)y;
-    double p2 = e64 / (t64 * t64), p3 = p2 / t64, p4 = p3 / t64, p5 = p4 / t64;
-    double e2 = e64 * e64;
-    as[0] += gs64 * p2;
-    as[1] += gs64 * (p3 * (gamma - e64));
-    as[2] += gs64 * (p4 * (gamma * gamma + e2 - (4.0 * gamma) * e64));
-    as[3] += gs64 * (p5 * (gamma * gamma * gamma - (11.0 * gamma * gamma) * e64 +
-                           (11.0 * gamma) * e2 - e2 * e64));
-  } else {
-    const CT u_ = ex * inv;
-    const CT gs = g + y;
-    const CT lt = fast_log(t, tab);
-    const double pw = (double)(gs * u_ * inv);
-    const double dres = (double)y - (double)gs * (double)u_;
-    const double term = (double)(y * eta) - (double)gs * (double)lt;
-    as[0] += dres;
-    as[1] += pw;
-    ag[0] += term;
-    double xd[QA];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) xd[q] = (double)xr[q];
-    if constexpr (MODE == SW_FINISH) {
-      // d pw / d eta = pw (gamma - e) / t
-      const double dpw = pw * (((double)g - (double)ex) * (double)inv);
-      as[NS] += dpw;
-#pragma unroll
-      for (int q = 0; q < Q; ++q) as[NS + 1 + q] += dpw * xd[q];
-    }
-#pragma unroll
-    for (int a = 0; a < Q; ++a) {
-      const double pa = pw * xd[a];
-      as[2 + a] += pa;
-      ag[1 + a] += xd[a] * dres;
-#pragma unroll
-      for (int b = a; b < Q; ++b) ag[1 + Q + tri_index(Q, a, b)] += pa * xd[b];
-    }
-  }
-}
-
 // One wave's part of a sweep: cells [cb, ce), subjects from s, CSR cursor ip
 // (first entry at or past cb).  Writes the wave's per-subject sums to psub
 // (slot s + wave) and its per-gene sums to pgene (row wave).  Everything it
@@ -612,7 +554,55 @@ __device__ __forceinline__ void sweep_wave(const CT* __restrict__ rows,
         for (int u = 0; u < U; ++u) {
           const uint32_t cell = base + (uint32_t)(lane + kWave * u);
           if (cell >= tend) continue;
-          add_cell<CT, MODE, NB>(xr[u], yv[u], bq, bint, c0, g, gamma, tab, as, ag);
+          // the cell's terms written out here: as an inlined helper taking
+          // the sums by pointer, p = 8 needed 256 VGPRs and spilled
+          const CT y = yv[u];
+          CT eta = xr[u][Q];  // log offset
+#pragma unroll
+          for (int q = 0; q < Q; ++q) eta += xr[u][q] * bq[q];
+          const CT ex = fast_exp(eta + c0, tab);
+          eta += bint;
+          const CT t = ex + g;
+          const CT inv = fast_rcp(t);
+          if constexpr (MODE == SW_LAPLACE) {
+            // (gamma + y) times the four derivative factors (pml.rs:1861-1874)
+            double e64 = (double)ex, t64 = e64 + gamma, gs64 = gamma + (double)y;
+            double p2 = e64 / (t64 * t64), p3 = p2 / t64, p4 = p3 / t64, p5 = p4 / t64;
+            double e2 = e64 * e64;
+            as[0] += gs64 * p2;
+            as[1] += gs64 * (p3 * (gamma - e64));
+            as[2] += gs64 * (p4 * (gamma * gamma + e2 - (4.0 * gamma) * e64));
+            as[3] += gs64 * (p5 * (gamma * gamma * gamma - (11.0 * gamma * gamma) * e64 +
+                                   (11.0 * gamma) * e2 - e2 * e64));
+          } else {
+            const CT u_ = ex * inv;
+            const CT gs = g + y;
+            const CT lt = fast_log(t, tab);
+            const double pw = (double)(gs * u_ * inv);
+            const double dres = (double)y - (double)gs * (double)u_;
+            const double term = (double)(y * eta) - (double)gs * (double)lt;
+            as[0] += dres;
+            as[1] += pw;
+            ag[0] += term;
+            double xd[QA];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) xd[q] = (double)xr[u][q];
+            if constexpr (MODE == SW_FINISH) {
+              // d pw / d eta = pw (gamma - e) / t
+              const double dpw = pw * (((double)g - (double)ex) * (double)inv);
+              as[NS] += dpw;
+#pragma unroll
+              for (int q = 0; q < Q; ++q) as[NS + 1 + q] += dpw * xd[q];
+            }
+#pragma unroll
+            for (int a = 0; a < Q; ++a) {
+              const double pa = pw * xd[a];
+              as[2 + a] += pa;
+              ag[1 + a] += xd[a] * dres;
+#pragma unroll
+              for (int b = a; b < Q; ++b) ag[1 + Q + tri_index(Q, a, b)] += pa * xd[b];
+            }
+          }
         }
       }
       // flush subject s: wave sum, slot s + wave
@@ -873,36 +863,49 @@ struct Fitter {
     __syncthreads();
   }
 
-  // This wave's part of a sweep on a design with zero-count tables, out of
-  // line so that the dense sweep's registers are allocated without it.
-  template <int MODE>
-  __device__ __noinline__ void wave_part_tabled(const double* bsrc, const double* lwsrc) {
+  // One pass over the gene's cells at (beta in LDS `bsrc`, log w `lwsrc`),
+  // leaving per-subject totals in wk.tsub and per-gene totals in wk.tgene.
+  // The dense and the tabled passes are separate out-of-line functions, so
+  // that neither's registers are allocated around a call to the other.
+  template <class CT, int MODE>
+  __device__ __forceinline__ void sweep(const double* bsrc, const double* lwsrc) {
+    if (dd.tabled)
+      sweep_tabled<CT, MODE>(bsrc, lwsrc);
+    else
+      sweep_dense<CT, MODE>(bsrc, lwsrc);
+  }
+
+  template <class CT, int MODE>
+  __device__ __noinline__ void sweep_dense(const double* bsrc, const double* lwsrc) {
+    const CT* rows = sizeof(CT) == 4 ? (const CT*)dd.row32 : (const CT*)dd.row64;
+    const CT* gy = sizeof(CT) == 4 ? (const CT*)dd.gy32 : (const CT*)dd.gy64;
+    sweep_wave<CT, MODE, NB>(rows, dd.gcell, gy, dd.fid, bsrc, lwsrc,
+                             (typename Lds<CT>::ptr)(&sh.ysc[wave][0]),
+                             (const lds_double*)&sh.f64tab[0], wk.psub, wk.pgene, sh.ip0[wave],
+                             g1, (int)sh.s0[wave], k, block_begin(wave), block_end(wave), icpt,
+                             gamma, lane, wave);
+    sweep_totals<CT, MODE>();
+  }
+
+  // The pass on a design with zero-count tables: always FP64 (the table
+  // lookups and the positive-count increments), counted as a sweep of CT.
+  template <class CT, int MODE>
+  __device__ __noinline__ void sweep_tabled(const double* bsrc, const double* lwsrc) {
     sweep_wave_tabled<MODE, NB>(dd.row64, dd.gcell, dd.gy64, dd.fid, dd.zgroup, dd.zsg, dd.zloose,
                                 dd.zsl, dd.zindex, dd.zcoef, bsrc, lwsrc,
                                 (const lds_double*)&sh.f64tab[0], wk.psub, wk.pgene,
                                 sh.ip0[wave], g1, (int)sh.s0[wave], k, block_begin(wave),
                                 block_end(wave), icpt, gamma, lane, wave);
+    sweep_totals<CT, MODE>();
   }
 
-  // One pass over the gene's cells at (beta in LDS `bsrc`, log w `lwsrc`),
-  // leaving per-subject totals in wk.tsub and per-gene totals in wk.tgene.
+  // Per-subject totals from the waves' partials, waves added in order;
+  // per-gene totals likewise; the sweep counters.
   template <class CT, int MODE>
-  __device__ __noinline__ void sweep(const double* bsrc, const double* lwsrc) {
+  __device__ __forceinline__ void sweep_totals() {
     constexpr int NSV = MODE == SW_LAPLACE ? 4 : (MODE == SW_FINISH ? D::NS3 : NS);
     constexpr int NGV = MODE == SW_LAPLACE ? 0 : NG;
-    if (dd.tabled) {
-      wave_part_tabled<MODE>(bsrc, lwsrc);
-    } else {
-      const CT* rows = sizeof(CT) == 4 ? (const CT*)dd.row32 : (const CT*)dd.row64;
-      const CT* gy = sizeof(CT) == 4 ? (const CT*)dd.gy32 : (const CT*)dd.gy64;
-      sweep_wave<CT, MODE, NB>(rows, dd.gcell, gy, dd.fid, bsrc, lwsrc,
-                               (typename Lds<CT>::ptr)(&sh.ysc[wave][0]),
-                               (const lds_double*)&sh.f64tab[0], wk.psub, wk.pgene,
-                               sh.ip0[wave], g1, (int)sh.s0[wave], k, block_begin(wave),
-                               block_end(wave), icpt, gamma, lane, wave);
-    }
     __syncthreads();
-    // per-subject totals, waves added in order; per-gene totals likewise
     for (int idx = tid; idx < k * NSV; idx += NT) {
       const int ss = idx / NSV, v = idx % NSV;
       const uint32_t f0 = dd.fid[ss], f1 = dd.fid[ss + 1];
