@@ -94,9 +94,14 @@ struct FitCounters {
   unsigned long long cells_f64, nnz_f64;  // summed over FP64 sweeps
 };
 
-// SW_FINISH: SW_FULL plus, per subject, the eta-derivatives of the weight
-// sum (sum of d pw / d eta, and that times each stored column), from which
-// the FP64 finish moves the log-determinant to the stepped point.
+// SW_FINISH: the first sweep of the FP64 finish.  Per subject the resid and
+// weight sums and the eta-derivatives of the weight sum (sum of d pw / d eta,
+// and that times each stored column), from which the finish moves the
+// log-determinant to the stepped point; per gene the log-likelihood and the
+// score.  The curvature blocks over the stored columns (pw x per subject,
+// pw x x' per gene) are not summed: the finish's one Newton step takes them
+// from the FP32 loop's last sweep, a point at most one small step away, where
+// they enter only the step's second-order terms.
 enum SweepMode { SW_FULL = 0, SW_FINISH = 1, SW_LAPLACE = 2 };
 
 // Cells per lane in one tile of a sweep (tile = 64 x U cells).
@@ -106,13 +111,22 @@ enum SweepMode { SW_FULL = 0, SW_FINISH = 1, SW_LAPLACE = 2 };
 #ifndef NH_TILE_F64
 #define NH_TILE_F64 1
 #endif
+#ifndef NH_TILE_F64_FINISH
+#define NH_TILE_F64_FINISH 2
+#endif
 constexpr int kTileF32 = NH_TILE_F32;
 constexpr int kTileF64 = NH_TILE_F64;
+// the finish sweep keeps fewer sums, so two cells per lane fit in registers
+// and their exp/log chains interleave
+constexpr int kTileF64Finish = NH_TILE_F64_FINISH;
+template <class CT, int MODE>
+constexpr int tile_cells() {
+  return sizeof(CT) == 4 ? kTileF32 : (MODE == SW_FINISH ? kTileF64Finish : kTileF64);
+}
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
 // floats in one wave's count staging row: a tile's counts in either type
 constexpr int kStageFloats =
-    kWave * (kTileF32 > 2 * kTileF64 ? kTileF32 : 2 * kTileF64) < kWave * 4
-        ? kWave * 4
-        : kWave * (kTileF32 > 2 * kTileF64 ? kTileF32 : 2 * kTileF64);
+    kWave * cmax(4, cmax(kTileF32, 2 * cmax(kTileF64, kTileF64Finish)));
 
 template <int NB>
 struct Dims {
@@ -120,9 +134,10 @@ struct Dims {
   static constexpr int R = Q + 1 <= 1 ? 1 : (Q + 1 <= 2 ? 2 : (Q + 1 <= 4 ? 4 : 8));
   static constexpr int QT = Q * (Q + 1) / 2;
   static constexpr int NS = 2 + Q;                // per subject: resid, weight, first[Q]
-  static constexpr int NS3 = NS + 1 + Q;          // SW_FINISH: + dweight, dweight x[Q]
-  static constexpr int NSL = NS3 > 4 ? NS3 : 4;   // slot width (Laplace keeps 4)
+  static constexpr int NF = 3 + Q;                // SW_FINISH: resid, weight, dweight, dweight x[Q]
+  static constexpr int NSL = NF > 4 ? NF : 4;     // slot width (Laplace keeps 4)
   static constexpr int NG = 1 + Q + QT;           // per gene: llp, score[Q], second[QT]
+  static constexpr int NGF = 1 + Q;               // SW_FINISH: llp, score[Q]
   static constexpr int SUB = 3 + NB;              // dw, vw, w, vwb[NB]
   static constexpr int NTRI = NB * (NB + 1) / 2;
 };
@@ -476,14 +491,13 @@ __device__ __forceinline__ void sweep_wave(const CT* __restrict__ rows,
                                         double gamma, int lane, int wave) {
   using D = Dims<NB>;
   constexpr int Q = D::Q, R = D::R, NS = D::NS, NSL = D::NSL, NG = D::NG;
-  constexpr int U = sizeof(CT) == 4 ? kTileF32 : kTileF64;
+  constexpr int U = tile_cells<CT, MODE>();
   const typename Glb<CT>::ptr grows = (typename Glb<CT>::ptr)rows;
   const typename Glb<CT>::ptr ggy = (typename Glb<CT>::ptr)gy;
   gconst_u32* const ggcell = (gconst_u32*)gcell;
   constexpr int TILE = kWave * U;
-  constexpr int NS3 = D::NS3;
-  constexpr int NSV = MODE == SW_LAPLACE ? 4 : (MODE == SW_FINISH ? NS3 : NS);
-  constexpr int NGV = MODE == SW_LAPLACE ? 0 : NG;
+  constexpr int NSV = MODE == SW_LAPLACE ? 4 : (MODE == SW_FINISH ? D::NF : NS);
+  constexpr int NGV = MODE == SW_LAPLACE ? 0 : (MODE == SW_FINISH ? D::NGF : NG);
   constexpr int QA = Q > 0 ? Q : 1;
   CT bq[QA];
 #pragma unroll
@@ -590,17 +604,21 @@ __device__ __forceinline__ void sweep_wave(const CT* __restrict__ rows,
             if constexpr (MODE == SW_FINISH) {
               // d pw / d eta = pw (gamma - e) / t
               const double dpw = pw * (((double)g - (double)ex) * (double)inv);
-              as[NS] += dpw;
+              as[2] += dpw;
 #pragma unroll
-              for (int q = 0; q < Q; ++q) as[NS + 1 + q] += dpw * xd[q];
-            }
+              for (int q = 0; q < Q; ++q) {
+                as[3 + q] += dpw * xd[q];
+                ag[1 + q] += xd[q] * dres;
+              }
+            } else {
 #pragma unroll
-            for (int a = 0; a < Q; ++a) {
-              const double pa = pw * xd[a];
-              as[2 + a] += pa;
-              ag[1 + a] += xd[a] * dres;
+              for (int a = 0; a < Q; ++a) {
+                const double pa = pw * xd[a];
+                as[2 + a] += pa;
+                ag[1 + a] += xd[a] * dres;
 #pragma unroll
-              for (int b = a; b < Q; ++b) ag[1 + Q + tri_index(Q, a, b)] += pa * xd[b];
+                for (int b = a; b < Q; ++b) ag[1 + Q + tri_index(Q, a, b)] += pa * xd[b];
+              }
             }
           }
         }
@@ -641,8 +659,8 @@ __device__ __forceinline__ void sweep_wave_tabled(
     uint32_t cb, uint32_t ce, int icpt, double gamma, int lane, int wave) {
   using D = Dims<NB>;
   constexpr int Q = D::Q, R = D::R, NS = D::NS, NSL = D::NSL, NG = D::NG;
-  constexpr int NSV = MODE == SW_LAPLACE ? 4 : (MODE == SW_FINISH ? D::NS3 : NS);
-  constexpr int NGV = MODE == SW_LAPLACE ? 0 : NG;
+  constexpr int NSV = MODE == SW_LAPLACE ? 4 : (MODE == SW_FINISH ? D::NF : NS);
+  constexpr int NGV = MODE == SW_LAPLACE ? 0 : (MODE == SW_FINISH ? D::NGF : NG);
   constexpr int QA = Q > 0 ? Q : 1;
   double bq[QA];
 #pragma unroll
@@ -664,17 +682,21 @@ __device__ __forceinline__ void sweep_wave_tabled(
       as[1] += p;
       ag[0] += tm;
       if constexpr (MODE == SW_FINISH) {
-        as[NS] += dp;
+        as[2] += dp;
 #pragma unroll
-        for (int q = 0; q < Q; ++q) as[NS + 1 + q] += dp * xd[q];
-      }
+        for (int q = 0; q < Q; ++q) {
+          as[3 + q] += dp * xd[q];
+          ag[1 + q] += xd[q] * d;
+        }
+      } else {
 #pragma unroll
-      for (int a = 0; a < Q; ++a) {
-        const double pa = p * xd[a];
-        as[2 + a] += pa;
-        ag[1 + a] += xd[a] * d;
+        for (int a = 0; a < Q; ++a) {
+          const double pa = p * xd[a];
+          as[2 + a] += pa;
+          ag[1 + a] += xd[a] * d;
 #pragma unroll
-        for (int b = a; b < Q; ++b) ag[1 + Q + tri_index(Q, a, b)] += pa * xd[b];
+          for (int b = a; b < Q; ++b) ag[1 + Q + tri_index(Q, a, b)] += pa * xd[b];
+        }
       }
     }
   };
@@ -903,8 +925,8 @@ struct Fitter {
   // per-gene totals likewise; the sweep counters.
   template <class CT, int MODE>
   __device__ __forceinline__ void sweep_totals() {
-    constexpr int NSV = MODE == SW_LAPLACE ? 4 : (MODE == SW_FINISH ? D::NS3 : NS);
-    constexpr int NGV = MODE == SW_LAPLACE ? 0 : NG;
+    constexpr int NSV = MODE == SW_LAPLACE ? 4 : (MODE == SW_FINISH ? D::NF : NS);
+    constexpr int NGV = MODE == SW_LAPLACE ? 0 : (MODE == SW_FINISH ? D::NGF : NG);
     __syncthreads();
     for (int idx = tid; idx < k * NSV; idx += NT) {
       const int ss = idx / NSV, v = idx % NSV;
@@ -945,10 +967,17 @@ struct Fitter {
   }
 
   // Gradient, curvature blocks and penalised log-likelihood of side `sd` from
-  // the sweep totals (pml.rs:1564-1589), log w in `lwsrc`.
-  __device__ __noinline__ void finalize(int sd, const double* lwsrc) {
+  // the sweep totals (pml.rs:1564-1589), log w in `lwsrc`.  hess_from < 0:
+  // every block from this sweep's sums.  hess_from = the other side: the
+  // sweep was an SW_FINISH sweep, and the blocks over the stored columns
+  // (each subject's vwb[j], the fixed-effect block between stored columns and
+  // between them and the intercept) are taken from that side; the gradient,
+  // the weights (vw, the intercept's blocks) and the log-likelihood are this
+  // sweep's.
+  __device__ __noinline__ void finalize(int sd, const double* lwsrc, int hess_from = -1) {
     const double* tot_y = dd.gtot + (size_t)gene * k;
     double* sub = side(sd);
+    const double* hsub = hess_from >= 0 ? side(hess_from) : nullptr;
     for (int s = tid; s < k; s += NT) {
       const double* t = wk.tsub + (size_t)s * NSL;
       const double lws = lwsrc[s];
@@ -958,16 +987,21 @@ struct Fitter {
       sub[s * SUB + 2] = w;
       sub[s * SUB + 3 + icpt] = gamma * t[1];
 #pragma unroll
-      for (int q = 0; q < Q; ++q) sub[s * SUB + 3 + qcol(q, icpt)] = gamma * t[2 + q];
+      for (int q = 0; q < Q; ++q) {
+        const int j = qcol(q, icpt);
+        sub[s * SUB + 3 + j] = hsub ? hsub[s * SUB + 3 + j] : gamma * t[2 + q];
+      }
       wk.llsub[s] = lws * tot_y[s] + (alpha * lws - lambda * w);
     }
     __syncthreads();
     // quantities summed over subjects: 0 resid, 1 weight, 2..1+Q first, 2+Q ll
     for (int e = wave; e < Q + 3; e += WAVES) {
-      double v;
-      if (e < 2 + Q)
+      double v = 0.0;
+      if (e < 2)
         v = subject_sum([&](int s) { return wk.tsub[(size_t)s * NSL + e]; });
-      else
+      else if (e < 2 + Q) {
+        if (hess_from < 0) v = subject_sum([&](int s) { return wk.tsub[(size_t)s * NSL + e]; });
+      } else
         v = subject_sum([&](int s) { return wk.llsub[s]; });
       if (lane == 0) {
         if (e == 0) {
@@ -976,8 +1010,9 @@ struct Fitter {
           sh.vb[sd][icpt * NB + icpt] = gamma * v;
         } else if (e < 2 + Q) {
           const int j = qcol(e - 2, icpt);
-          sh.vb[sd][icpt * NB + j] = gamma * v;
-          sh.vb[sd][j * NB + icpt] = gamma * v;
+          const double h = hess_from >= 0 ? sh.vb[hess_from][icpt * NB + j] : gamma * v;
+          sh.vb[sd][icpt * NB + j] = h;
+          sh.vb[sd][j * NB + icpt] = h;
         } else {
           sh.ll[sd] = wk.tgene[0] + v;
         }
@@ -994,8 +1029,9 @@ struct Fitter {
           ++a;
         }
         const int b = a + r;
-        const double v = gamma * wk.tgene[1 + Q + tq];
         const int ja = qcol(a, icpt), jb = qcol(b, icpt);
+        const double v =
+            hess_from >= 0 ? sh.vb[hess_from][ja * NB + jb] : gamma * wk.tgene[1 + Q + tq];
         sh.vb[sd][ja * NB + jb] = v;
         sh.vb[sd][jb * NB + ja] = v;
       }
@@ -1223,9 +1259,9 @@ struct Fitter {
     if (wave == 0) {
       const double acc = subject_sum([&](int s) {
         const double* t = wk.tsub + (size_t)s * NSL;
-        double dw = t[D::NS] * (wk.step_lw[s] + sh.step_beta[icpt]);
+        double dw = t[2] * (wk.step_lw[s] + sh.step_beta[icpt]);
 #pragma unroll
-        for (int q = 0; q < Q; ++q) dw += t[D::NS + 1 + q] * sh.step_beta[qcol(q, icpt)];
+        for (int q = 0; q < Q; ++q) dw += t[3 + q] * sh.step_beta[qcol(q, icpt)];
         const double w1 = exp(wk.lw[s] + wk.step_lw[s]);
         const double vw = sub[s * SUB + 1] + gamma * dw + lambda * (w1 - sub[s * SUB + 2]);
         return log(fabs(vw));
@@ -1238,15 +1274,18 @@ struct Fitter {
 
   // FP64 finish from the current point (newton_finish, pml.rs:807-1090):
   // Newton steps without damping, log-likelihood and log-determinant at the
-  // stepped point.  Returns false when a step worsens the objective beyond
-  // rounding or leaves it not finite, or the budget runs out.
-  __device__ bool finish(const FitKnobs& kn, FitResult& out) {
-    int cur = 0;
-    if (kn.decrement_stop)
+  // stepped point.  `c32`: the side holding the FP32 loop's last sweep.
+  // Returns false when a step worsens the objective beyond rounding or leaves
+  // it not finite, or the budget runs out.
+  __device__ bool finish(const FitKnobs& kn, FitResult& out, int c32) {
+    int cur = 1 - c32;
+    if (kn.decrement_stop) {
       sweep<double, SW_FINISH>(sh.beta, wk.lw);
-    else
+      finalize(cur, wk.lw, c32);
+    } else {
       sweep<double, SW_FULL>(sh.beta, wk.lw);
-    finalize(cur, wk.lw);
+      finalize(cur, wk.lw);
+    }
     for (int it = 1; it <= kn.max_iter; ++it) {
       const double ll_prev = sh.ll[cur];
       newton_step(cur);
@@ -1352,10 +1391,10 @@ struct Fitter {
     bool done = false;
     if (mode == 0) {
       FitResult f32r{};
-      newton_loop<float>(kn, kn.noise_rel, f32r);
+      const int c32 = newton_loop<float>(kn, kn.noise_rel, f32r);
       r.iterations = f32r.iterations;
       r.backtracks = f32r.backtracks;
-      done = finish(kn, r);
+      done = finish(kn, r, c32);
     }
     if (!done) {
       FitResult e{};

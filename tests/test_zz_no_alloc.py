@@ -61,7 +61,9 @@ def test_noalloc_matches_solve_on_two_calls(shape):
 def test_noalloc_refuses_a_batch_beyond_its_scratch():
     w, rb = resident(2, 14_400, 180, 8, "equal", "config3", 33)
     try:
-        n = 400
+        # at 180 subjects and p = 8 a fit's work area (~72 KB) lives in global
+        # memory: 1,000 of them need about twice the 32 MiB array
+        n = 1000
         with pytest.raises(nh.InvalidArgument, match="scratch"):
             rb.solve_noalloc(np.zeros(n, dtype=np.int64), 0.3, 1.5, np.zeros(w.n_coef))
         # the handle and the entry point still work afterwards
