@@ -778,6 +778,10 @@ void run_stage_one(Prepared& P, Resident& R, int host_threads, uint64_t& rounds)
     pl.convergence = res[i].failed ? 0 : 1;
     pl.sigma = res[i].x[nb];
     pl.gamma = res[i].x[nb + 1];
+    pl.beta_one = pl.beta_start;
+    bool beta_fin = true;
+    for (size_t j = 0; j < nb; ++j) beta_fin = beta_fin && std::isfinite(res[i].x[j]);
+    if (beta_fin) pl.beta_one.assign(res[i].x.begin(), res[i].x.begin() + nb);
     if (P.method == NH_METHOD_HL) {
       pl.refit = 1;
     } else {
@@ -930,12 +934,15 @@ void run_stage_two_plain(Prepared& P, Resident& R, int waves, nebula_run_stats_t
         } else {
           c.point(subject, cell);
         }
-        req.push_back(DevRequest{(uint32_t)i, (uint32_t)c.point_order(), subject, cell});
-        if (!S[i].warm_beta.empty()) {
+        const bool cold = S[i].warm_beta.empty();
+        req.push_back(DevRequest{(uint32_t)i,
+                                 (uint32_t)c.point_order() | (cold ? kOrdMeanStart : 0u),
+                                 subject, cell});
+        if (!cold) {
           bi.insert(bi.end(), S[i].warm_beta.begin(), S[i].warm_beta.end());
           lwi.insert(lwi.end(), S[i].warm_lw.begin(), S[i].warm_lw.end());
         } else {
-          std::vector<double> b0 = P.plans[i].beta_start;
+          std::vector<double> b0 = P.plans[i].beta_one;
           b0[P.intercept] -= subject / 2.0;
           bi.insert(bi.end(), b0.begin(), b0.end());
           lwi.insert(lwi.end(), k, 0.0);

@@ -25,7 +25,8 @@ struct GenePlan {
   unsigned ord = 1;
   double sigma = 0, gamma = 0;  // stage one's variance components
   int convergence = 1;
-  std::vector<double> beta_start;
+  std::vector<double> beta_start;  // stage three's start (the reference's)
+  std::vector<double> beta_one;    // stage two's cold fits: stage one's fixed effects
   int refit = 1;  // 0 none, 1 both, 2 subject only
   bool searched = false, found = false, failed = false;
   double v[2] = {0, 0};  // stage-two minimiser

@@ -36,7 +36,7 @@ void pml_batch_kernel(
   const double alpha = 1.0 / (exps - 1.0);
   const double lambda = 1.0 / (sqrt(exps) * (exps - 1.0));
   Fitter<NB, WAVES> f(dd, sh, wk, ctr, r.gene, r.cell, alpha, lambda);
-  FitResult res = f.run(mode, r.ord, kn);
+  FitResult res = f.run(mode, r.ord & ~kOrdMeanStart, kn, false, (r.ord & kOrdMeanStart) != 0);
   if (threadIdx.x == 0) {
     DevReply o;
     o.ll = res.ll;

@@ -9,9 +9,13 @@
 
 namespace nh {
 
+// DevRequest::ord flag: a cold stage-two fit, whose log w start is set from
+// the subjects' Poisson-limit optimum at the start's beta (Fitter::run).
+constexpr uint32_t kOrdMeanStart = 1u << 16;
+
 struct DevRequest {
   uint32_t gene;
-  uint32_t ord;
+  uint32_t ord;  // Laplace order, | kOrdMeanStart
   double subject;  // nebula's sigma[0]
   double cell;     // NB size gamma
 };

@@ -121,7 +121,8 @@ void stage_two_kernel(PersistentArgs A) {
       const double alpha = 1.0 / (exps - 1.0);
       const double lambda = 1.0 / (sqrt(exps) * (exps - 1.0));
       Fitter<NB, WAVES> f(dd, sh, wk, A.counters, S.gene, cell, alpha, lambda);
-      const FitResult r = f.run(0, (unsigned)ctl.point_order(), A.knobs, ctl.evals > 0);
+      const FitResult r =
+          f.run(0, (unsigned)ctl.point_order(), A.knobs, ctl.evals > 0, !have_warm);
       // finite point and the lgamma tail over the count histogram
       double part = 0.0;
       bool fin = true;
@@ -337,7 +338,7 @@ static std::shared_ptr<StageTwoPlan> build_stage_two_plan(const Prepared& P, Res
     d.kind = pl.refit == 1 ? kSearchBoth : kSearchSubject;
     d.start[0] = pl.sigma;
     d.start[1] = pl.gamma;
-    for (size_t j = 0; j < nb; ++j) d.beta_start[j] = pl.beta_start[j];
+    for (size_t j = 0; j < nb; ++j) d.beta_start[j] = pl.beta_one[j];
     d.tail_off = tv.size();
     d.tail_len = (uint32_t)gi.tail.value.size();
     tv.insert(tv.end(), gi.tail.value.begin(), gi.tail.value.end());

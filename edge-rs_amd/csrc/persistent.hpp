@@ -14,7 +14,7 @@ struct DevSearch {
   uint32_t ord;          // Laplace order of the first pass
   uint32_t kind;         // SearchKind: 2 both components, 1 the subject variance alone
   double start[2];       // stage one's (sigma, gamma); gamma is held for kind 1
-  double beta_start[8];  // fixed effects every cold fit starts from
+  double beta_start[8];  // fixed effects every cold fit starts from (stage one's)
   uint64_t tail_off;     // count histogram slice (values != 1, 2)
   uint32_t tail_len;
   double n_pos, n_one, n_two;

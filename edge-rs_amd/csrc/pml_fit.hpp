@@ -640,6 +640,50 @@ __device__ __forceinline__ void sweep_wave(const CT* __restrict__ rows,
   }
 }
 
+// One wave's part of the pass that starts a cold fit's log w: per subject of
+// its block [cb, ce), sum over the cells of exp(x'beta + log offset) in FP32
+// (a start value only), written to slot s + wave.
+template <int NB>
+__device__ __forceinline__ void sweep_wave_mean(const float* __restrict__ rows,
+                                                const uint32_t* __restrict__ fid,
+                                                const double* __restrict__ bsrc,
+                                                double* __restrict__ psub, int s, int k,
+                                                uint32_t cb, uint32_t ce, int icpt, int lane,
+                                                int wave) {
+  using D = Dims<NB>;
+  constexpr int Q = D::Q, R = D::R, NSL = D::NSL;
+  constexpr int QA = Q > 0 ? Q : 1;
+  constexpr int U = 4;
+  const gconst_float* grows = (const gconst_float*)rows;
+  float bq[QA];
+#pragma unroll
+  for (int q = 0; q < QA; ++q) bq[q] = q < Q ? (float)bsrc[q < icpt ? q : q + 1] : 0.0f;
+  const float bint = (float)bsrc[icpt];
+  if (cb >= ce) return;
+  for (; s < k && fid[s] < ce; ++s) {
+    const uint32_t lo = max(fid[s], cb), hi = min(fid[s + 1], ce);
+    float acc = 0.0f;
+    for (uint32_t base = lo; base < hi; base += kWave * U) {
+      float xr[U][R];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t c = base + (uint32_t)(lane + kWave * u);
+        load_row<float, R>(grows, c < hi ? c : hi - 1, xr[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        float eta = xr[u][Q] + bint;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) eta += xr[u][q] * bq[q];
+        const float e = fast_exp(eta, nullptr);
+        if (base + (uint32_t)(lane + kWave * u) < hi) acc += e;
+      }
+    }
+    const double r = wave_sum((double)acc);
+    if (lane == 0) psub[(size_t)(s + wave) * NSL] = r;
+  }
+}
+
 // One wave's part of a sweep on a design with zero-count tables, FP64
 // throughout (src/sc/pml.rs:1510-1560, :1869-1890): per subject of its block
 // [cb, ce), the zero-count sums of the subject's groups whose first cell lies
@@ -919,6 +963,28 @@ struct Fitter {
                                 sh.ip0[wave], g1, (int)sh.s0[wave], k, block_begin(wave),
                                 block_end(wave), icpt, gamma, lane, wave);
     sweep_totals<CT, MODE>();
+  }
+
+  // Cold start of log w (dense designs): at the start's beta, each subject's
+  // w solves the Poisson limit of its stationarity condition,
+  // Y_s + alpha = (M_s + lambda) w_s with M_s = sum of exp(eta) over its
+  // cells, the exact optimum when every cell's exp(eta) is small against
+  // gamma; the Newton loop starts there instead of at log w = 0.
+  __device__ __noinline__ void mean_start() {
+    sweep_wave_mean<NB>(dd.row32, dd.fid, sh.beta, wk.psub, (int)sh.s0[wave], k,
+                        block_begin(wave), block_end(wave), icpt, lane, wave);
+    __syncthreads();
+    const double* tot_y = dd.gtot + (size_t)gene * k;
+    for (int s = tid; s < k; s += NT) {
+      const uint32_t f0 = dd.fid[s], f1 = dd.fid[s + 1];
+      const int w0 = (int)(f0 / blk), w1 = (int)((f1 - 1) / blk);
+      double m = 0.0;
+      for (int w = w0; w <= w1; ++w) m += wk.psub[(size_t)(s + w) * NSL];
+      const double v = log((tot_y[s] + alpha) / (m + lambda));
+      wk.lw[s] = isfinite(v) ? v : 0.0;
+    }
+    if (tid == 0) atomicAdd(&counters->cells_f32, (unsigned long long)dd.n_cells);
+    __syncthreads();
   }
 
   // Per-subject totals from the waves' partials, waves added in order;
@@ -1383,11 +1449,15 @@ struct Fitter {
   // log w in wk.lw on entry.  same_gene: this workgroup's previous fit was of
   // the same gene, so setup()'s per-wave CSR cursors, the zeroed count rows
   // (every sweep leaves them zero) and the FP64 tables are still in LDS.
-  __device__ FitResult run(int mode, unsigned ord, const FitKnobs& kn, bool same_gene = false) {
+  // cold: a stage-two fit with no earlier fit of the gene to start from; on a
+  // dense design its log w starts at mean_start()'s values.
+  __device__ FitResult run(int mode, unsigned ord, const FitKnobs& kn, bool same_gene = false,
+                           bool cold = false) {
     FitResult r{};
     r.finish_steps = 0;
     r.exact = 0;
     if (!same_gene) setup();
+    if (cold && mode == 0 && !dd.tabled) mean_start();
     bool done = false;
     if (mode == 0) {
       FitResult f32r{};
