@@ -91,7 +91,7 @@ static hipError_t launch_batch_nb(const BatchLaunch& L, hipStream_t st) {
 
 __global__ __launch_bounds__(256) void f64_math_kernel(const double* x, size_t n, int which,
                                                         double* out) {
-  __shared__ double tab[82];
+  __shared__ double tab[kF64TabDoubles];
   fill_f64_tables(tab, (int)threadIdx.x);
   __syncthreads();
   const lds_double* t = (const lds_double*)tab;

@@ -124,6 +124,8 @@ constexpr int tile_cells() {
   return sizeof(CT) == 4 ? kTileF32 : (MODE == SW_FINISH ? kTileF64Finish : kTileF64);
 }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
+// doubles in the FP64 sweeps' exp and log tables (fast_exp, fast_log)
+constexpr int kF64TabDoubles = 64 + 2 * 129;
 // floats in one wave's count staging row: a tile's counts in either type
 constexpr int kStageFloats =
     kWave * cmax(4, cmax(kTileF32, 2 * cmax(kTileF64, kTileF64Finish)));
@@ -209,7 +211,7 @@ struct FitLds {
   int perm[NB];
   double ll[2];
   double scal[8];
-  double f64tab[82];  // tables of the FP64 sweep's exp and log (Fitter::setup)
+  double f64tab[kF64TabDoubles];  // the FP64 sweep's exp and log tables (Fitter::setup)
   uint64_t ip0[WAVES];
   uint32_t s0[WAVES];
   float ysc[WAVES][kStageFloats];  // count staging: one tile's counts per wave
@@ -404,52 +406,50 @@ constexpr double kF32Trust = 1e-6;
 
 // FP64 exp, log and reciprocal of the FP64 sweeps, with short dependency
 // chains and no special-case branches (the sweep's arguments are finite;
-// overflow gives inf, underflow 0, NaN stays NaN).  Tables in LDS:
-//   [0, 16)   2^(j/16)
-//   [16, 49)  1 / c_j,  c_j = 1 + j/32, j = 0..32
-//   [49, 82)  log c_j (0 for j = 32, whose binade is moved up by one)
-constexpr int kTabExp = 0, kTabInvc = 16, kTabLogc = 49;
+// overflow gives inf, underflow 0, NaN stays NaN).  Tables in LDS
+// (kF64TabDoubles doubles, filled by fill_f64_tables):
+//   [0, 64)            2^(j/64)
+//   [64 + 2j, +2)      1 / c_j and log c_j, c_j = 1 + j/128, j = 0..128
+//                      (log c_128 = 0: its binade is moved up by one)
+constexpr int kTabExp = 0, kTabLog = 64;
 constexpr double kLn2Hi = 6.93147180369123816490e-01;  // 32 significant bits
 constexpr double kLn2Lo = 1.90821492927058770002e-10;
 
-// exp(x) = 2^(i/16) e^r, i = rint(16 x / ln 2), |r| <= ln 2 / 32; e^r - 1 by
-// its Taylor series to r^7 (truncation < 2e-18 relative); <= 1 ulp.
+// exp(x) = 2^(i/64) e^r, i = rint(64 x / ln 2), |r| <= ln 2 / 128; e^r - 1 by
+// its Taylor series to r^5 (truncation < 4e-17 relative); within 2 ulp.
 __device__ __forceinline__ double fast_exp(double x, const lds_double* tab) {
-  const double kd = __builtin_rint(x * 23.083120654223414);  // 16 / ln 2
-  double r = __builtin_fma(-kd, kLn2Hi * 0.0625, x);
-  r = __builtin_fma(-kd, kLn2Lo * 0.0625, r);
+  const double kd = __builtin_rint(x * 92.332482616893657);  // 64 / ln 2
+  double r = __builtin_fma(-kd, kLn2Hi * 0.015625, x);
+  r = __builtin_fma(-kd, kLn2Lo * 0.015625, r);
   const int i = (int)kd;  // saturates for |x| beyond the double range of exp
-  const double s = tab[kTabExp + (i & 15)];
-  double q = __builtin_fma(r, 1.0 / 5040, 1.0 / 720);
-  q = __builtin_fma(r, q, 1.0 / 120);
-  q = __builtin_fma(r, q, 1.0 / 24);
+  const double s = tab[kTabExp + (i & 63)];
+  double q = __builtin_fma(r, 1.0 / 120, 1.0 / 24);
   q = __builtin_fma(r, q, 1.0 / 6);
   q = __builtin_fma(r, q, 0.5);
   const double em1 = __builtin_fma(r * r, q, r);
-  return __builtin_ldexp(__builtin_fma(s, em1, s), i >> 4);
+  return __builtin_ldexp(__builtin_fma(s, em1, s), i >> 6);
 }
 
-// log(t), t > 0: t = 2^e m, m in [1, 2); c_j the nearest multiple of 1/32,
-// r = m / c_j - 1 (one fma), |r| <= 1/64; log(1 + r) by its series to r^8
-// (truncation < 1e-17); absolute error about 1 ulp of the result's larger
+// log(t), t > 0: t = 2^e m, m in [1, 2); c_j the nearest multiple of 1/128,
+// r = m / c_j - 1 (one fma), |r| <= 1/256; log(1 + r) by its series to r^6
+// (truncation < 3e-18); absolute error about 1 ulp of the result's larger
 // part.  inf and NaN pass through.
 __device__ __forceinline__ double fast_log(double t, const lds_double* tab) {
   const uint64_t b = __builtin_bit_cast(uint64_t, t);
   const uint32_t hi = (uint32_t)(b >> 32);
-  const uint32_t j = (((hi >> 14) & 63u) + 1u) >> 1;
-  const int e = (int)(hi >> 20) - 1023 + (int)(j >> 5);
+  const uint32_t j = (((hi >> 12) & 255u) + 1u) >> 1;
+  const int e = (int)(hi >> 20) - 1023 + (int)(j >> 7);
   const double m = __builtin_bit_cast(
       double, ((uint64_t)((hi & 0x000FFFFFu) | 0x3FF00000u) << 32) | (uint32_t)b);
-  const double r = __builtin_fma(m, tab[kTabInvc + j], -1.0);
-  double q = __builtin_fma(r, -1.0 / 8, 1.0 / 7);
-  q = __builtin_fma(r, q, -1.0 / 6);
-  q = __builtin_fma(r, q, 1.0 / 5);
+  const double invc = tab[kTabLog + 2 * j], logc = tab[kTabLog + 2 * j + 1];
+  const double r = __builtin_fma(m, invc, -1.0);
+  double q = __builtin_fma(r, -1.0 / 6, 1.0 / 5);
   q = __builtin_fma(r, q, -1.0 / 4);
   q = __builtin_fma(r, q, 1.0 / 3);
   q = __builtin_fma(r, q, -0.5);
   const double lp = __builtin_fma(r * r, q, r);
   const double ed = (double)e;
-  const double res = __builtin_fma(ed, kLn2Hi, tab[kTabLogc + j]) + __builtin_fma(ed, kLn2Lo, lp);
+  const double res = __builtin_fma(ed, kLn2Hi, logc) + __builtin_fma(ed, kLn2Lo, lp);
   return t < __builtin_inf() ? res : t;
 }
 
@@ -462,13 +462,15 @@ __device__ __forceinline__ double fast_rcp(double t) {
   return __builtin_fma(r, e, r);
 }
 
-// Fills the tables above (one workgroup's copy; threads 0..32).
+// Fills the tables above (one workgroup's copy; threads 0..128).
 __device__ __forceinline__ void fill_f64_tables(double* tab, int tid) {
-  if (tid < 16) tab[kTabExp + tid] = exp2((double)tid * 0.0625);
-  if (tid < 33) {
-    const double invc = 1.0 / (1.0 + (double)tid * 0.03125);
-    tab[kTabInvc + tid] = invc;
-    tab[kTabLogc + tid] = tid == 32 ? 0.0 : -log(invc);
+  if (tid < 64) tab[kTabExp + tid] = exp2((double)tid * 0.015625);
+  if (tid < 129) {
+    // log c_j as -log of the rounded 1 / c_j, whose rounding then cancels
+    // against the one r = m / c_j - 1 carries
+    const double invc = 1.0 / (1.0 + (double)tid * 0.0078125);
+    tab[kTabLog + 2 * tid] = invc;
+    tab[kTabLog + 2 * tid + 1] = tid == 128 ? 0.0 : -log(invc);
   }
 }
 

@@ -215,7 +215,7 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_pass_kernel(DeviceData dd, co
   __shared__ double ysc[WAVES][kWave];
   __shared__ uint64_t ip0[WAVES];
   __shared__ uint32_t s0[WAVES];
-  __shared__ double f64tab[82];  // the FP64 sweep's exp / log tables
+  __shared__ double f64tab[kF64TabDoubles];  // the FP64 sweep's exp / log tables
   const int units = chunks * WAVES;
   // wave-uniform: keeps the request's values in scalar registers
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);

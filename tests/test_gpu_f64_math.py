@@ -2,7 +2,7 @@
 reciprocal and the device objective's log Gamma, run on the device through
 the C ABI, against numpy's libm and scipy: exp
 within 2 ulp, reciprocal within 1 ulp, log within 2 ulp of its larger part
-(relative accuracy kept within 1/256 of t = 1, where log 1 = 0 exactly),
+(relative accuracy kept within 1/512 of t = 1, where log 1 = 0 exactly),
 overflow, underflow, inf and NaN as libm gives them.  Marked gpu."""
 import numpy as np
 import pytest
@@ -19,7 +19,7 @@ def _ulp_err(got, ref):
 def test_exp_matches_libm():
     rng = np.random.default_rng(11)
     x = np.concatenate([rng.uniform(-708.0, 709.0, 400_000), rng.uniform(-40.0, 20.0, 400_000),
-                        np.arange(-64, 65) * np.log(2.0) / 16.0,  # table knots
+                        np.arange(-256, 257) * np.log(2.0) / 64.0,  # table knots
                         [0.0, -0.0, 1e-300, -1e-300, 1.0, -1.0]])
     got = nh.selftest_f64_math(x, 0)
     ref = np.exp(x)
@@ -41,9 +41,9 @@ def test_log_matches_libm():
     t = np.concatenate([np.exp(rng.uniform(-700.0, 700.0, 400_000)),
                         rng.uniform(1e-4, 50.0, 400_000),
                         1.0 + rng.uniform(-1e-3, 1e-3, 100_000)])
-    # mantissas at and beside the interval edges 1 + (j +- 1/2) / 32
-    j = np.arange(0, 33)
-    edges = 1.0 + (j - 0.5) / 32.0
+    # mantissas at and beside the interval edges 1 + (j +- 1/2) / 128
+    j = np.arange(0, 129)
+    edges = 1.0 + (j - 0.5) / 128.0
     t = np.concatenate([t, edges, np.nextafter(edges, 0), np.nextafter(edges, 3),
                         edges * 2.0 ** 40, edges * 2.0 ** -40, [1.0, 2.0, 0.5, 1e-300, 1e300]])
     got = nh.selftest_f64_math(t, 1)
@@ -51,8 +51,8 @@ def test_log_matches_libm():
     big = np.maximum(np.abs(ref), np.abs(np.round(np.log2(t))) * np.log(2.0))
     err = np.abs(got - ref) / np.spacing(np.maximum(big, 0.5))
     assert err.max() <= 2.0, (err.max(), t[np.argmax(err)])
-    # within 1/256 of 1 the table point is c = 1 (log c = 0): relative accuracy
-    near = np.abs(t - 1.0) < 1.0 / 256
+    # within 1/512 of 1 the table point is c = 1 (log c = 0): relative accuracy
+    near = np.abs(t - 1.0) < 1.0 / 512
     assert _ulp_err(got[near & (t != 1.0)], ref[near & (t != 1.0)]).max() <= 4.0
     assert nh.selftest_f64_math(np.array([1.0]), 1)[0] == 0.0
     e = nh.selftest_f64_math(np.array([np.inf, np.nan]), 1)
