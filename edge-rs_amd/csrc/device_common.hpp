@@ -26,6 +26,25 @@ __device__ __forceinline__ double wave_max(double v) {
 
 __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __ballot(p); }
 
+// Values every lane of the wave holds alike, moved to scalar registers
+// (arguments of out-of-line device functions arrive in vector registers, so
+// the compiler cannot tell they are uniform); uniform base pointers let
+// loads use a scalar base with 32-bit per-lane offsets.
+__device__ __forceinline__ uint32_t uni(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uni(uint64_t v) {
+  return ((uint64_t)uni((uint32_t)(v >> 32)) << 32) | uni((uint32_t)v);
+}
+__device__ __forceinline__ double uni(double v) {
+  return __builtin_bit_cast(double, uni(__builtin_bit_cast(uint64_t, v)));
+}
+template <class T>
+__device__ __forceinline__ T* uni(T* p) {
+  return (T*)uni((uint64_t)p);
+}
+
 __device__ __forceinline__ int popc64(uint64_t m) { return __popcll(m); }
 
 // Makes this wave's LDS writes visible to its other lanes before they read.

@@ -352,6 +352,16 @@ __device__ __forceinline__ void load_row(P base, uint32_t cell, CT* out) {
   }
 }
 
+// The same from a uniform base and a 32-bit byte offset (cell * R * size),
+// so the load takes the base in scalar registers and one offset register.
+typedef __attribute__((address_space(1))) const char gconst_char;
+template <class CT, int R>
+__device__ __forceinline__ void load_row_at(gconst_char* base, uint32_t cell, CT* out) {
+  const uint32_t off = cell * (uint32_t)(R * sizeof(CT));
+  load_row<CT, R>(reinterpret_cast<__attribute__((address_space(1))) const CT*>(base + off), 0u,
+                  out);
+}
+
 // index of (a, b), a <= b, in the packed upper triangle of a Q x Q matrix
 __host__ __device__ constexpr int tri_index(int q, int a, int b) {
   return a * q - a * (a - 1) / 2 + (b - a);
@@ -494,9 +504,22 @@ __device__ __forceinline__ void sweep_wave(const CT* __restrict__ rows,
   using D = Dims<NB>;
   constexpr int Q = D::Q, R = D::R, NS = D::NS, NSL = D::NSL, NG = D::NG;
   constexpr int U = tile_cells<CT, MODE>();
-  const typename Glb<CT>::ptr grows = (typename Glb<CT>::ptr)rows;
-  const typename Glb<CT>::ptr ggy = (typename Glb<CT>::ptr)gy;
-  gconst_u32* const ggcell = (gconst_u32*)gcell;
+  // every argument but lane is the wave's own: scalar registers
+  gconst_char* const grows = (gconst_char*)uni(rows);
+  fid = uni(fid);
+  bsrc = uni(bsrc);
+  lwsrc = uni(lwsrc);
+  psub = uni(psub);
+  pgene = uni(pgene);
+  ip = uni(ip);
+  g1 = uni(g1);
+  s = uni(s);
+  k = uni(k);
+  cb = uni(cb);
+  ce = uni(ce);
+  icpt = uni(icpt);
+  gamma = uni(gamma);
+  wave = uni(wave);
   constexpr int TILE = kWave * U;
   constexpr int NSV = MODE == SW_LAPLACE ? 4 : (MODE == SW_FINISH ? D::NF : NS);
   constexpr int NGV = MODE == SW_LAPLACE ? 0 : (MODE == SW_FINISH ? D::NGF : NG);
@@ -514,16 +537,30 @@ __device__ __forceinline__ void sweep_wave(const CT* __restrict__ rows,
     // issued unconditionally at an in-bounds index (the last entry when past
     // the row; buffers hold at least one entry) and masked afterwards, so the
     // compiler can count outstanding loads and wait only for the ones needed.
+    // Indices are 32-bit, relative to wbase = min(ip, last entry): the
+    // cursor rel and the row end n_rel (one gene's entries fit 32 bits).
     const uint64_t e_last = g1 > 0 ? g1 - 1 : 0;
+    const uint64_t wbase = ip < e_last ? ip : e_last;
+    gconst_char* const ggcell = (gconst_char*)((gconst_u32*)uni(gcell) + wbase);
+    gconst_char* const ggy = (gconst_char*)((typename Glb<CT>::ptr)uni(gy) + wbase);
+    // entry ec of the window: 32-bit byte offsets from the scalar bases
+    auto entry = [&](uint32_t ec, uint32_t& c, CT& y) {
+      c = *(gconst_u32*)(ggcell + (ec << 2));
+      y = *(typename Glb<CT>::ptr)(ggy + ec * (uint32_t)sizeof(CT));
+    };
+    const uint32_t n_rel = (uint32_t)(g1 - wbase);
+    const uint32_t c_last = n_rel > 0 ? n_rel - 1 : 0;
+    uint32_t rel = (uint32_t)(ip - wbase);
     uint32_t wc[U];
     CT wy[U];
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-      const uint64_t e = ip + (uint64_t)(lane + kWave * j);
-      const uint64_t ec = e < g1 ? e : e_last;
-      const uint32_t c = ggcell[ec];
-      const CT y = ggy[ec];
-      wc[j] = e < g1 ? c : 0xffffffffu;
+      const uint32_t e = rel + (uint32_t)(lane + kWave * j);
+      const uint32_t ec = e < n_rel ? e : c_last;
+      uint32_t c;
+      CT y;
+      entry(ec, c, y);
+      wc[j] = e < n_rel ? c : 0xffffffffu;
       wy[j] = y;
     }
     for (; s < k && fid[s] < ce; ++s) {
@@ -539,24 +576,25 @@ __device__ __forceinline__ void sweep_wave(const CT* __restrict__ rows,
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const uint32_t cell = base + (uint32_t)(lane + kWave * u);
-          load_row<CT, R>(grows, cell < tend ? cell : tend - 1, xr[u]);
+          load_row_at<CT, R>(grows, cell < tend ? cell : tend - 1, xr[u]);
         }
         // the window's entries inside [base, tend) form a prefix of it
-        uint64_t m = 0;
+        uint32_t m = 0;
 #pragma unroll
         for (int j = 0; j < U; ++j) {
           const bool in = wc[j] < tend;
           if (in) ys[wc[j] - base] = wy[j];
-          m += (uint64_t)popc64(wave_ballot(in));
+          m += (uint32_t)popc64(wave_ballot(in));
         }
-        ip += m;
+        rel += m;
 #pragma unroll
         for (int j = 0; j < U; ++j) {
-          const uint64_t e = ip + (uint64_t)(lane + kWave * j);
-          const uint64_t ec = e < g1 ? e : e_last;
-          const uint32_t c = ggcell[ec];
-          const CT y = ggy[ec];
-          wc[j] = e < g1 ? c : 0xffffffffu;
+          const uint32_t e = rel + (uint32_t)(lane + kWave * j);
+          const uint32_t ec = e < n_rel ? e : c_last;
+          uint32_t c;
+          CT y;
+          entry(ec, c, y);
+          wc[j] = e < n_rel ? c : 0xffffffffu;
           wy[j] = y;
         }
         __builtin_amdgcn_wave_barrier();
