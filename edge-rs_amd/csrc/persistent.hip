@@ -40,6 +40,18 @@
 
 namespace nh {
 
+// Largest first-order move of any coordinate (beta, log w) a warm start takes
+// from the anchor fit; beyond it the start stays at the anchor's optimum.
+// Starts moved by at most kSkipCap skip the FP32 loop.
+#ifndef NH_PREDICT_CAP
+#define NH_PREDICT_CAP 0.25
+#endif
+#ifndef NH_SKIP_CAP
+#define NH_SKIP_CAP 0.003
+#endif
+constexpr double kPredictCap = NH_PREDICT_CAP;
+constexpr double kSkipCap = NH_SKIP_CAP;
+
 // Profile objective on the device, same arithmetic as host_common.hpp's.
 __device__ static double device_objective(double subject, double cell, const FitResult& r,
                                           double n_cells, double k, double tail, double n_pos,
@@ -83,9 +95,11 @@ void stage_two_kernel(PersistentArgs A) {
   __shared__ FitLds<NB, WAVES> sh;
   __shared__ SearchControl ctl;
   __shared__ double warm_beta[NB];
+  __shared__ double warm_jb[2 * NB];  // d beta* / d (subject, gamma) at the anchor
+  __shared__ double warm_vb[NB * NB];  // the anchor fit's fixed-effect curvature block
   __shared__ double red[NT / 64];
-  __shared__ int s_slot, s_ok;
-  __shared__ double s_subject, s_cell;
+  __shared__ int s_slot, s_ok, s_pred;
+  __shared__ double s_subject, s_cell, s_anchor[2];
   extern __shared__ double nh_s2_dyn[];
   const int tid = threadIdx.x;
   const DeviceData& dd = A.dd;
@@ -94,6 +108,8 @@ void stage_two_kernel(PersistentArgs A) {
   double* base = A.work_in_lds ? nh_s2_dyn : A.scratch + (size_t)blockIdx.x * A.stride;
   wk.carve(base, k, WAVES);
   double* warm_lw = base + FitWork<NB>::doubles(k, WAVES);
+  double* warm_jl = warm_lw + k;  // 2 x k: d log w* / d (subject, gamma)
+  double* warm_vwb = warm_jl + 2 * (size_t)k;  // k x NB: the anchor fit's vwb blocks
   for (;;) {
     if (tid == 0) s_slot = (int)atomicAdd(A.queue, 1u);
     __syncthreads();
@@ -101,6 +117,7 @@ void stage_two_kernel(PersistentArgs A) {
     __syncthreads();
     if (slot >= A.n_search) break;
     const DevSearch S = A.search[A.order[slot]];
+    if (tid == 0) s_pred = 0;
     if (tid == 0)
       ctl_begin(&ctl, &A.search[A.order[slot]],
                 A.pred ? A.pred + 2 * (size_t)A.order[slot] : nullptr, &A);
@@ -110,19 +127,38 @@ void stage_two_kernel(PersistentArgs A) {
       __syncthreads();
       const double subject = s_subject, cell = s_cell;
       const int have_warm = ctl.have_warm;
+      // A warm fit starts from the anchor fit's optimum moved to first order
+      // to this (subject, gamma), unless some coordinate would move by more
+      // than kPredictCap (where the linear model is not trusted).
+      const double dsub = subject - s_anchor[0], dcell = cell - s_anchor[1];
+      double move = 0.0;
+      if (have_warm && s_pred) {
+        if (tid < NB) move = fabs(warm_jb[tid] * dsub + warm_jb[NB + tid] * dcell);
+        for (int s = tid; s < k; s += NT)
+          move = fmax(move, fabs(warm_jl[s] * dsub + warm_jl[k + s] * dcell));
+      }
+      const bool pred = have_warm && s_pred && !__syncthreads_or(move > kPredictCap);
+      const bool near = pred && !__syncthreads_or(move > kSkipCap);
       if (tid < NB) {
         double b = S.beta_start[tid];
         if (tid == (int)A.intercept) b -= subject / 2.0;
-        sh.beta[tid] = have_warm ? warm_beta[tid] : b;
+        if (have_warm)
+          b = warm_beta[tid] + (pred ? warm_jb[tid] * dsub + warm_jb[NB + tid] * dcell : 0.0);
+        sh.beta[tid] = b;
       }
-      for (int s = tid; s < k; s += NT) wk.lw[s] = have_warm ? warm_lw[s] : 0.0;
+      for (int s = tid; s < k; s += NT)
+        wk.lw[s] = have_warm ? warm_lw[s] + (pred ? warm_jl[s] * dsub + warm_jl[k + s] * dcell
+                                                  : 0.0)
+                             : 0.0;
       __syncthreads();
       const double exps = exp(subject);
       const double alpha = 1.0 / (exps - 1.0);
       const double lambda = 1.0 / (sqrt(exps) * (exps - 1.0));
       Fitter<NB, WAVES> f(dd, sh, wk, A.counters, S.gene, cell, alpha, lambda);
-      const FitResult r =
-          f.run(0, (unsigned)ctl.point_order(), A.knobs, ctl.evals > 0, !have_warm);
+      // a predicted start goes straight to the FP64 finish, with the anchor
+      // fit's curvature blocks
+      const FitResult r = f.run(0, (unsigned)ctl.point_order(), A.knobs, ctl.evals > 0,
+                                !have_warm, near ? warm_vwb : nullptr, near ? warm_vb : nullptr);
       // finite point and the lgamma tail over the count histogram
       double part = 0.0;
       bool fin = true;
@@ -151,6 +187,19 @@ void stage_two_kernel(PersistentArgs A) {
       if (s_ok) {
         if (tid < NB) warm_beta[tid] = sh.beta[tid];
         for (int s = tid; s < k; s += NT) warm_lw[s] = wk.lw[s];
+        const bool can = A.predict && r.predictable;
+        if (can) {
+          f.sensitivity(r.side, subject, warm_jb, warm_jl);
+          const double* hs = f.side(r.side);
+          for (int i = tid; i < k * NB; i += NT)
+            warm_vwb[i] = hs[(i / NB) * Fitter<NB, WAVES>::SUB + 3 + i % NB];
+          if (tid < NB * NB) warm_vb[tid] = sh.vb[r.side][tid];
+        }
+        if (tid == 0) {
+          s_pred = can ? 1 : 0;
+          s_anchor[0] = subject;
+          s_anchor[1] = cell;
+        }
       }
       __syncthreads();
     }
@@ -254,7 +303,11 @@ __global__ void reset_queue_kernel(unsigned* q) {
   if (threadIdx.x == 0) *q = 0u;
 }
 
-size_t persistent_stride(int nb, int k, int waves) { return fit_work_doubles(nb, k, waves) + (size_t)k; }
+// per workgroup: the fit's work arrays, the anchor's log w, its two
+// sensitivity columns and its vwb curvature blocks
+size_t persistent_stride(int nb, int k, int waves) {
+  return fit_work_doubles(nb, k, waves) + (3 + (size_t)nb) * (size_t)k;
+}
 
 template <int NB, int WAVES>
 static hipError_t launch_t(const PersistentArgs& A, int grid, hipStream_t st) {
@@ -394,6 +447,7 @@ static std::shared_ptr<StageTwoPlan> build_stage_two_plan(const Prepared& P, Res
   A.pred = nullptr;
   A.warm_step = 0.0;
   A.warm_xtol = 0.0;
+  A.predict = 1;
   if (learned) {
     // features of the searched genes in search order; the network's
     // predictions land in dpred before each replay

@@ -46,6 +46,8 @@ struct PersistentArgs {
   const float* pred;      // learned starts: n_search x (log sigma^2, log phi), or null
   double warm_step;       // warm search's initial step relative to the predicted point
   double warm_xtol;       // warm search's rhoend / rhobeg
+  int predict;            // warm inner fits start at the anchor's first-order
+                          // prediction (Fitter::sensitivity)
 };
 
 size_t persistent_stride(int nb, int k, int waves);

@@ -95,10 +95,11 @@ struct FitCounters {
 };
 
 // SW_FINISH: the first sweep of the FP64 finish.  Per subject the resid and
-// weight sums and the eta-derivatives of the weight sum (sum of d pw / d eta,
+// weight sums, the eta-derivatives of the weight sum (sum of d pw / d eta,
 // and that times each stored column), from which the finish moves the
-// log-determinant to the stepped point; per gene the log-likelihood and the
-// score.  The curvature blocks over the stored columns (pw x per subject,
+// log-determinant to the stepped point, and the gamma-derivative of the resid
+// sum; per gene the log-likelihood, the score and the gamma-derivative of the
+// score (Fitter::sensitivity).  The curvature blocks over the stored columns (pw x per subject,
 // pw x x' per gene) are not summed: the finish's one Newton step takes them
 // from the FP32 loop's last sweep, a point at most one small step away, where
 // they enter only the step's second-order terms.
@@ -136,10 +137,11 @@ struct Dims {
   static constexpr int R = Q + 1 <= 1 ? 1 : (Q + 1 <= 2 ? 2 : (Q + 1 <= 4 ? 4 : 8));
   static constexpr int QT = Q * (Q + 1) / 2;
   static constexpr int NS = 2 + Q;                // per subject: resid, weight, first[Q]
-  static constexpr int NF = 3 + Q;                // SW_FINISH: resid, weight, dweight, dweight x[Q]
+  static constexpr int NF = 4 + Q;                // SW_FINISH: resid, weight, dweight, dresid/dgamma,
+                                                  //   dweight x[Q]
   static constexpr int NSL = NF > 4 ? NF : 4;     // slot width (Laplace keeps 4)
   static constexpr int NG = 1 + Q + QT;           // per gene: llp, score[Q], second[QT]
-  static constexpr int NGF = 1 + Q;               // SW_FINISH: llp, score[Q]
+  static constexpr int NGF = 1 + 2 * Q;           // SW_FINISH: llp, score[Q], dscore/dgamma[Q]
   static constexpr int SUB = 3 + NB;              // dw, vw, w, vwb[NB]
   static constexpr int NTRI = NB * (NB + 1) / 2;
 };
@@ -206,6 +208,7 @@ struct FitLds {
   double vb[2][NB * NB];
   double vb2[NB * NB];
   double rhs[NB];
+  double rhs2[2][NB];  // Fitter::sensitivity's right-hand sides
   double lm[NB * NB];  // LDL' work
   double lb[NB];
   int perm[NB];
@@ -221,6 +224,9 @@ struct FitLds {
 struct FitResult {
   double ll, ll_prev, log_det, second;
   int iterations, backtracks, finish_steps, exact;  // exact: f64 Newton loop ran
+  int side;         // side holding the curvature blocks of the fit's last step
+  int predictable;  // the SW_FINISH sums of that step are in the totals
+                    // (Fitter::sensitivity may run)
 };
 
 // ---------------------------------------------------------------------------
@@ -642,13 +648,16 @@ __device__ __forceinline__ void sweep_wave(const CT* __restrict__ rows,
 #pragma unroll
             for (int q = 0; q < Q; ++q) xd[q] = (double)xr[u][q];
             if constexpr (MODE == SW_FINISH) {
-              // d pw / d eta = pw (gamma - e) / t
+              // d pw / d eta = pw (gamma - e) / t; d resid / d gamma = (y - e) e / t^2
               const double dpw = pw * (((double)g - (double)ex) * (double)inv);
+              const double dgam = ((double)y - (double)ex) * ((double)u_ * (double)inv);
               as[2] += dpw;
+              as[3] += dgam;
 #pragma unroll
               for (int q = 0; q < Q; ++q) {
-                as[3 + q] += dpw * xd[q];
+                as[4 + q] += dpw * xd[q];
                 ag[1 + q] += xd[q] * dres;
+                ag[1 + Q + q] += xd[q] * dgam;
               }
             } else {
 #pragma unroll
@@ -766,10 +775,12 @@ __device__ __forceinline__ void sweep_wave_tabled(
       as[1] += p;
       ag[0] += tm;
       if constexpr (MODE == SW_FINISH) {
+        // the gamma-derivatives (slot 3, per-gene 1 + Q + q) stay zero: the
+        // tabled fits make no start predictions
         as[2] += dp;
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-          as[3 + q] += dp * xd[q];
+          as[4 + q] += dp * xd[q];
           ag[1 + q] += xd[q] * d;
         }
       } else {
@@ -1073,17 +1084,19 @@ struct Fitter {
   }
 
   // Gradient, curvature blocks and penalised log-likelihood of side `sd` from
-  // the sweep totals (pml.rs:1564-1589), log w in `lwsrc`.  hess_from < 0:
-  // every block from this sweep's sums.  hess_from = the other side: the
-  // sweep was an SW_FINISH sweep, and the blocks over the stored columns
-  // (each subject's vwb[j], the fixed-effect block between stored columns and
-  // between them and the intercept) are taken from that side; the gradient,
-  // the weights (vw, the intercept's blocks) and the log-likelihood are this
-  // sweep's.
-  __device__ __noinline__ void finalize(int sd, const double* lwsrc, int hess_from = -1) {
+  // the sweep totals (pml.rs:1564-1589), log w in `lwsrc`.  hvb null: every
+  // block from this sweep's sums.  Otherwise the sweep was an SW_FINISH sweep,
+  // and the blocks over the stored columns (each subject's vwb[j], at
+  // hvwb[s * hstride + j], and the fixed-effect block hvb between stored
+  // columns and between them and the intercept) are taken from there; the
+  // gradient, the weights (vw, the intercept's blocks) and the log-likelihood
+  // are this sweep's.
+  __device__ __noinline__ void finalize(int sd, const double* lwsrc,
+                                        const double* hvwb = nullptr, int hstride = 0,
+                                        const double* hvb = nullptr) {
     const double* tot_y = dd.gtot + (size_t)gene * k;
     double* sub = side(sd);
-    const double* hsub = hess_from >= 0 ? side(hess_from) : nullptr;
+    const double* hsub = hvwb;
     for (int s = tid; s < k; s += NT) {
       const double* t = wk.tsub + (size_t)s * NSL;
       const double lws = lwsrc[s];
@@ -1095,7 +1108,7 @@ struct Fitter {
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
         const int j = qcol(q, icpt);
-        sub[s * SUB + 3 + j] = hsub ? hsub[s * SUB + 3 + j] : gamma * t[2 + q];
+        sub[s * SUB + 3 + j] = hsub ? hsub[s * hstride + j] : gamma * t[2 + q];
       }
       wk.llsub[s] = lws * tot_y[s] + (alpha * lws - lambda * w);
     }
@@ -1106,7 +1119,7 @@ struct Fitter {
       if (e < 2)
         v = subject_sum([&](int s) { return wk.tsub[(size_t)s * NSL + e]; });
       else if (e < 2 + Q) {
-        if (hess_from < 0) v = subject_sum([&](int s) { return wk.tsub[(size_t)s * NSL + e]; });
+        if (!hvb) v = subject_sum([&](int s) { return wk.tsub[(size_t)s * NSL + e]; });
       } else
         v = subject_sum([&](int s) { return wk.llsub[s]; });
       if (lane == 0) {
@@ -1116,7 +1129,7 @@ struct Fitter {
           sh.vb[sd][icpt * NB + icpt] = gamma * v;
         } else if (e < 2 + Q) {
           const int j = qcol(e - 2, icpt);
-          const double h = hess_from >= 0 ? sh.vb[hess_from][icpt * NB + j] : gamma * v;
+          const double h = hvb ? hvb[icpt * NB + j] : gamma * v;
           sh.vb[sd][icpt * NB + j] = h;
           sh.vb[sd][j * NB + icpt] = h;
         } else {
@@ -1136,8 +1149,7 @@ struct Fitter {
         }
         const int b = a + r;
         const int ja = qcol(a, icpt), jb = qcol(b, icpt);
-        const double v =
-            hess_from >= 0 ? sh.vb[hess_from][ja * NB + jb] : gamma * wk.tgene[1 + Q + tq];
+        const double v = hvb ? hvb[ja * NB + jb] : gamma * wk.tgene[1 + Q + tq];
         sh.vb[sd][ja * NB + jb] = v;
         sh.vb[sd][jb * NB + ja] = v;
       }
@@ -1367,7 +1379,7 @@ struct Fitter {
         const double* t = wk.tsub + (size_t)s * NSL;
         double dw = t[2] * (wk.step_lw[s] + sh.step_beta[icpt]);
 #pragma unroll
-        for (int q = 0; q < Q; ++q) dw += t[3 + q] * sh.step_beta[qcol(q, icpt)];
+        for (int q = 0; q < Q; ++q) dw += t[4 + q] * sh.step_beta[qcol(q, icpt)];
         const double w1 = exp(wk.lw[s] + wk.step_lw[s]);
         const double vw = sub[s * SUB + 1] + gamma * dw + lambda * (w1 - sub[s * SUB + 2]);
         return log(fabs(vw));
@@ -1380,14 +1392,16 @@ struct Fitter {
 
   // FP64 finish from the current point (newton_finish, pml.rs:807-1090):
   // Newton steps without damping, log-likelihood and log-determinant at the
-  // stepped point.  `c32`: the side holding the FP32 loop's last sweep.
-  // Returns false when a step worsens the objective beyond rounding or leaves
-  // it not finite, or the budget runs out.
-  __device__ bool finish(const FitKnobs& kn, FitResult& out, int c32) {
-    int cur = 1 - c32;
+  // stepped point.  `cur`: the side the first sweep goes to; with the
+  // decrement stop the stored-column curvature blocks of the first step come
+  // from hvwb (stride hstride) and hvb (see finalize).  Returns false when a
+  // step worsens the objective beyond rounding or leaves it not finite, or
+  // the budget runs out.
+  __device__ bool finish(const FitKnobs& kn, FitResult& out, int cur, const double* hvwb,
+                         int hstride, const double* hvb) {
     if (kn.decrement_stop) {
       sweep<double, SW_FINISH>(sh.beta, wk.lw);
-      finalize(cur, wk.lw, c32);
+      finalize(cur, wk.lw, hvwb, hstride, hvb);
     } else {
       sweep<double, SW_FULL>(sh.beta, wk.lw);
       finalize(cur, wk.lw);
@@ -1411,6 +1425,8 @@ struct Fitter {
           out.log_det = ld;
           out.finish_steps = it;
           out.second = 0.0;
+          out.side = cur;
+          out.predictable = 1;
           return true;  // the information at the start point stays in vb2
         }
       }
@@ -1440,6 +1456,59 @@ struct Fitter {
       }
     }
     return false;
+  }
+
+  // First-order change of the optimum (beta, log w) with the variance
+  // components (implicit function theorem: the gradient g stays zero, so
+  // d x* / d theta = I^-1 d g / d theta with I the information), at the point
+  // whose curvature blocks side `sd` holds and whose SW_FINISH sums are in the
+  // totals.  Column 0 per unit of `subject` (through alpha and lambda, whose
+  // derivatives are closed forms), column 1 per unit of gamma (the sweep's
+  // d resid / d gamma sums).  jb: 2 x NB (LDS), jl: 2 x k.  The same Schur
+  // complement and pivoted LDL' solve as newton_step.
+  __device__ __noinline__ void sensitivity(int sd, double subject, double* jb, double* jl) {
+    const double* sub = side(sd);
+    const double exps = exp(subject), em1 = exps - 1.0;
+    const double dalpha = -exps / (em1 * em1);
+    const double dlambda = -lambda * (0.5 + exps / em1);
+    for (int s = tid; s < k; s += NT) {
+      wk.step_lw[s] = dalpha - dlambda * sub[s * SUB + 2];
+      wk.damp_lw[s] = wk.tsub[(size_t)s * NSL + 3];
+    }
+    schur(sd, false);  // ends with a barrier
+    for (int e = wave; e < 2 * NB; e += WAVES) {
+      const int col = e / NB, j = e % NB;
+      const double* r = col == 0 ? wk.step_lw : wk.damp_lw;
+      const double acc =
+          subject_sum([&](int s) { return sub[s * SUB + 3 + j] * (r[s] / sub[s * SUB + 1]); });
+      double rb = 0.0;
+      if (col == 1) {
+        if (j == icpt)
+          rb = subject_sum([&](int s) { return wk.damp_lw[s]; });
+        else
+          rb = wk.tgene[1 + Q + (j < icpt ? j : j - 1)];
+      }
+      if (lane == 0) sh.rhs2[col][j] = rb - acc;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      for (int col = 0; col < 2; ++col) {
+        for (int i = 0; i < NB * NB; ++i) sh.lm[i] = sh.vb2[i];
+        for (int i = 0; i < NB; ++i) sh.lb[i] = sh.rhs2[col][i];
+        ldlt_solve_lds<NB>(sh.lm, sh.lb, sh.perm);
+        for (int i = 0; i < NB; ++i) jb[col * NB + i] = sh.lb[i];
+      }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < 2 * k; idx += NT) {
+      const int col = idx / k, s = idx % k;
+      const double* r = col == 0 ? wk.step_lw : wk.damp_lw;
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < NB; ++j) acc += sub[s * SUB + 3 + j] * jb[col * NB + j];
+      jl[idx] = (r[s] - acc) / sub[s * SUB + 1];
+    }
+    __syncthreads();
   }
 
   // log-det from the last Newton step's vw (pml.rs:1767-1771)
@@ -1490,21 +1559,30 @@ struct Fitter {
   // the same gene, so setup()'s per-wave CSR cursors, the zeroed count rows
   // (every sweep leaves them zero) and the FP64 tables are still in LDS.
   // cold: a stage-two fit with no earlier fit of the gene to start from; on a
-  // dense design its log w starts at mean_start()'s values.
+  // dense design its log w starts at mean_start()'s values.  near_vwb /
+  // near_vb (stage two, decrement stop): the start was predicted to first
+  // order from a fit whose stored-column curvature blocks these are (vwb
+  // per subject, stride NB); the FP32 loop is skipped and the FP64 finish
+  // takes its first step with those blocks -- a start that close to the
+  // optimum passes the finish's decrement test, and one that does not
+  // continues with the finish's measured FP64 steps.
   __device__ FitResult run(int mode, unsigned ord, const FitKnobs& kn, bool same_gene = false,
-                           bool cold = false) {
+                           bool cold = false, const double* near_vwb = nullptr,
+                           const double* near_vb = nullptr) {
     FitResult r{};
     r.finish_steps = 0;
     r.exact = 0;
     if (!same_gene) setup();
     if (cold && mode == 0 && !dd.tabled) mean_start();
     bool done = false;
-    if (mode == 0) {
+    if (mode == 0 && near_vb && kn.decrement_stop) {
+      done = finish(kn, r, 0, near_vwb, NB, near_vb);
+    } else if (mode == 0) {
       FitResult f32r{};
       const int c32 = newton_loop<float>(kn, kn.noise_rel, f32r);
       r.iterations = f32r.iterations;
       r.backtracks = f32r.backtracks;
-      done = finish(kn, r, c32);
+      done = finish(kn, r, 1 - c32, side(c32) + 3, SUB, sh.vb[c32]);
     }
     if (!done) {
       FitResult e{};
@@ -1518,6 +1596,9 @@ struct Fitter {
       (void)cur;
     }
     r.second = ord > 1 ? laplace(ord) : 0.0;
+    // the Laplace sweep overwrote the totals; tabled sweeps leave the
+    // gamma-derivatives out
+    if (ord > 1 || dd.tabled) r.predictable = 0;
     return r;
   }
 };
