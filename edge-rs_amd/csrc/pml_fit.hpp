@@ -431,6 +431,16 @@ constexpr int kTabExp = 0, kTabLog = 64;
 constexpr double kLn2Hi = 6.93147180369123816490e-01;  // 32 significant bits
 constexpr double kLn2Lo = 1.90821492927058770002e-10;
 
+// a * b + c with c a constant held in scalar registers.  Left to itself the
+// compiler keeps such constants in vector registers and, to use the
+// two-operand v_fmac_f64 that overwrites its addend, copies one before every
+// multiply-add of a polynomial; v_fma_f64 reads the constant in place.
+__device__ __forceinline__ double fma_sk(double a, double b, double c) {
+  double d;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+  return d;
+}
+
 // exp(x) = 2^(i/64) e^r, i = rint(64 x / ln 2), |r| <= ln 2 / 128; e^r - 1 by
 // its Taylor series to r^5 (truncation < 4e-17 relative); within 2 ulp.
 __device__ __forceinline__ double fast_exp(double x, const lds_double* tab) {
@@ -439,8 +449,8 @@ __device__ __forceinline__ double fast_exp(double x, const lds_double* tab) {
   r = __builtin_fma(-kd, kLn2Lo * 0.015625, r);
   const int i = (int)kd;  // saturates for |x| beyond the double range of exp
   const double s = tab[kTabExp + (i & 63)];
-  double q = __builtin_fma(r, 1.0 / 120, 1.0 / 24);
-  q = __builtin_fma(r, q, 1.0 / 6);
+  double q = fma_sk(r, 1.0 / 120, 1.0 / 24);
+  q = fma_sk(r, q, 1.0 / 6);
   q = __builtin_fma(r, q, 0.5);
   const double em1 = __builtin_fma(r * r, q, r);
   return __builtin_ldexp(__builtin_fma(s, em1, s), i >> 6);
@@ -459,9 +469,9 @@ __device__ __forceinline__ double fast_log(double t, const lds_double* tab) {
       double, ((uint64_t)((hi & 0x000FFFFFu) | 0x3FF00000u) << 32) | (uint32_t)b);
   const double invc = tab[kTabLog + 2 * j], logc = tab[kTabLog + 2 * j + 1];
   const double r = __builtin_fma(m, invc, -1.0);
-  double q = __builtin_fma(r, -1.0 / 6, 1.0 / 5);
-  q = __builtin_fma(r, q, -1.0 / 4);
-  q = __builtin_fma(r, q, 1.0 / 3);
+  double q = fma_sk(r, -1.0 / 6, 1.0 / 5);
+  q = fma_sk(r, q, -1.0 / 4);
+  q = fma_sk(r, q, 1.0 / 3);
   q = __builtin_fma(r, q, -0.5);
   const double lp = __builtin_fma(r * r, q, r);
   const double ed = (double)e;
