@@ -122,6 +122,9 @@ void stage_two_kernel(PersistentArgs A) {
       ctl_begin(&ctl, &A.search[A.order[slot]],
                 A.pred ? A.pred + 2 * (size_t)A.order[slot] : nullptr, &A);
     __syncthreads();
+    // one fit object per gene (it lives in private memory: its members are
+    // written once here, the variance components once per evaluation)
+    Fitter<NB, WAVES> f(dd, sh, wk, A.counters, S.gene, 0.0, 0.0, 0.0);
     while (ctl.active) {
       if (tid == 0) ctl.point(s_subject, s_cell);
       __syncthreads();
@@ -154,7 +157,9 @@ void stage_two_kernel(PersistentArgs A) {
       const double exps = exp(subject);
       const double alpha = 1.0 / (exps - 1.0);
       const double lambda = 1.0 / (sqrt(exps) * (exps - 1.0));
-      Fitter<NB, WAVES> f(dd, sh, wk, A.counters, S.gene, cell, alpha, lambda);
+      f.gamma = cell;
+      f.alpha = alpha;
+      f.lambda = lambda;
       // a predicted start goes straight to the FP64 finish, with the anchor
       // fit's curvature blocks
       const FitResult r = f.run(0, (unsigned)ctl.point_order(), A.knobs, ctl.evals > 0,

@@ -946,6 +946,11 @@ struct Fitter {
 
   __device__ double* side(int sd) const { return wk.subs + (size_t)sd * k * SUB; }
 
+  // the FP64 exp / log tables (filled by setup); the per-subject steps use
+  // the sweeps' exp and log, whose short code keeps these steps' register
+  // use, and so their call-site saves, small
+  __device__ const lds_double* tab() const { return (const lds_double*)&sh.f64tab[0]; }
+
   // design column of stored column q
   __device__ static constexpr int qcol(int q, int ic) { return q < ic ? q : q + 1; }
 
@@ -1110,7 +1115,7 @@ struct Fitter {
     for (int s = tid; s < k; s += NT) {
       const double* t = wk.tsub + (size_t)s * NSL;
       const double lws = lwsrc[s];
-      const double w = exp(lws);
+      const double w = fast_exp(lws, tab());
       sub[s * SUB + 0] = t[0] + (alpha - lambda * w);
       sub[s * SUB + 1] = gamma * t[1] + lambda * w;
       sub[s * SUB + 2] = w;
@@ -1390,9 +1395,9 @@ struct Fitter {
         double dw = t[2] * (wk.step_lw[s] + sh.step_beta[icpt]);
 #pragma unroll
         for (int q = 0; q < Q; ++q) dw += t[4 + q] * sh.step_beta[qcol(q, icpt)];
-        const double w1 = exp(wk.lw[s] + wk.step_lw[s]);
+        const double w1 = fast_exp(wk.lw[s] + wk.step_lw[s], tab());
         const double vw = sub[s * SUB + 1] + gamma * dw + lambda * (w1 - sub[s * SUB + 2]);
-        return log(fabs(vw));
+        return fast_log(fabs(vw), tab());
       });
       if (lane == 0) sh.scal[5] = acc;
     }
