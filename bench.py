@@ -394,7 +394,12 @@ def main():
                     counters["SQ_ACTIVE_INST_VALU"] / counters["SQ_WAVE_CYCLES"], 4)
         fracs = {"hbm_algorithmic": roof["frac"], "dram": roof.get("dram_frac", 0.0),
                  "valu_issue": roof.get("valu_issue_frac", 0.0)}
-        roof["limiter"] = max(fracs, key=fracs.get)
+        top = max(fracs, key=fracs.get)
+        # a resource below 0.7 of its peak over the launch does not bind it:
+        # then the launch is the latency of the per-gene chains of dependent
+        # evaluations (serial steps between sweeps, the slowest searches)
+        roof["limiter"] = top if fracs[top] >= 0.7 else "latency"
+        roof["utilisation"] = fracs
         roof["counters_source"] = ("rocprofv3 --pmc passes over a child run of this bench "
                                    "(same workload, learned engine, last stage_two_kernel "
                                    "dispatch); VALU issue = SQ_INSTS_VALU / (256 CUs x 2.4 GHz "
