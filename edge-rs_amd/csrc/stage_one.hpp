@@ -411,7 +411,10 @@ inline S1Task stage_one_search(std::vector<double> start, std::vector<double> lo
   const size_t n = start.size();
   std::vector<double> x0(n);
   for (size_t j = 0; j < n; ++j) x0[j] = std::min(std::max(start[j], lo[j]), hi[j]);
-  EvalRes r0 = co_await Eval{EvalReq{x0, false}};
+  // the Newton descent's first step needs the start's Hessian: ask for it
+  // with the start's value (the same sums, so the same bits) rather than in
+  // a second round
+  EvalRes r0 = co_await Eval{EvalReq{x0, !tables}};
   const double f0 = r0.f;
   if (!std::isfinite(f0)) {
     out->x = x0;
@@ -448,7 +451,11 @@ inline S1Task stage_one_search(std::vector<double> start, std::vector<double> lo
     if (!tables) {
       // newton_marginal
       std::vector<double> x = xs;
-      EvalRes e = co_await Eval{EvalReq{x, true}};
+      EvalRes e;
+      if (attempt == 0)
+        e = r0;  // the start, evaluated with its Hessian
+      else
+        e = co_await Eval{EvalReq{x, true}};
       double f = e.f;
       std::vector<double> g = e.g, h = e.h;
       bool ok = all_finite(f, g, h);
