@@ -29,6 +29,9 @@
 
 namespace nh {
 
+// Cells per lane in one tile of a pass's walk (tile = 64 x kPtmgTile cells).
+constexpr int kPtmgTile = 2;
+
 template <int NB>
 struct PtmgDims {
   static constexpr int NTRI = NB * (NB + 1) / 2;
@@ -101,7 +104,7 @@ struct PtmgWalk {
   uint64_t g0, g1;
   uint32_t blk;
   int k, unit, wave, lane;
-  double (*ysc)[kWave];
+  double (*ysc)[kWave * kPtmgTile];
   uint64_t* ip0;
   uint32_t* s0;
 
@@ -139,55 +142,83 @@ struct PtmgWalk {
       ip0[wave] = ip;
       s0[wave] = (uint32_t)a;
     }
+#pragma unroll
+    for (int u = 0; u < kPtmgTile; ++u) ysc[wave][lane + kWave * u] = 0.0;
     __syncthreads();
   }
 
   // Calls begin(s) before and flush(s) after each subject segment of the
   // unit's block, uniformly across the wave, and cell(s, cell, y, row) for
   // every cell of the segment (active lanes only; row = the cell's packed
-  // design row, R doubles).  The next 64 CSR
-  // entries are held in registers and reloaded as soon as a tile has taken
-  // its own, so their loads overlap the tile's arithmetic; every load is
-  // issued at a clamped in-bounds index through global-address-space pointers.
+  // design row, R doubles).  Tiles of 64 x kPtmgTile cells, kPtmgTile per
+  // lane, as in the stage-two sweeps: the rows come from a scalar base with
+  // 32-bit offsets, and the next 64 x kPtmgTile CSR entries (32-bit indices
+  // relative to the wave's first entry) are held in registers and reloaded as
+  // soon as a tile has scattered its own into the wave's LDS row, so their
+  // loads overlap the tile's arithmetic; every load is issued at a clamped
+  // in-bounds index.  The LDS row starts zeroed (setup) and each tile zeroes
+  // what it read.
   template <class Begin, class Cell, class Flush>
   __device__ void walk(Begin&& begin, Cell&& cell, Flush&& flush) {
     constexpr int R = Dims<NB>::R;
+    constexpr int U = kPtmgTile;
+    constexpr uint32_t TILE = kWave * U;
     const uint32_t cb = bb(), ce = be();
     if (cb >= ce) return;
-    const gconst_double* rows = (const gconst_double*)dd.row64;
-    const gconst_u32* gcell = (const gconst_u32*)dd.gcell;
-    const gconst_double* gy = (const gconst_double*)dd.gy64;
+    gconst_char* const rows = (gconst_char*)dd.row64;
     const uint64_t e_last = g1 > 0 ? g1 - 1 : 0;
-    uint64_t ip = ip0[wave];
-    int s = (int)s0[wave];
-    uint64_t e = ip + (uint64_t)lane;
-    uint64_t ec = e < g1 ? e : e_last;
-    uint32_t wc = gcell[ec];
-    double wy = gy[ec];
-    wc = e < g1 ? wc : 0xffffffffu;
-    for (; s < k && dd.fid[s] < ce; ++s) {
+    const uint64_t ip = uni(ip0[wave]);
+    const uint64_t wbase = ip < e_last ? ip : e_last;
+    gconst_char* const gc = (gconst_char*)((gconst_u32*)dd.gcell + wbase);
+    gconst_char* const gyb = (gconst_char*)((gconst_double*)dd.gy64 + wbase);
+    const uint32_t n_rel = (uint32_t)(g1 - wbase);
+    const uint32_t c_last = n_rel > 0 ? n_rel - 1 : 0;
+    uint32_t rel = (uint32_t)(ip - wbase);
+    auto entry = [&](uint32_t e, uint32_t& c, double& y) {
+      const uint32_t ec = e < n_rel ? e : c_last;
+      c = *(gconst_u32*)(gc + (ec << 2));
+      y = *(gconst_double*)(gyb + (ec << 3));
+      c = e < n_rel ? c : 0xffffffffu;
+    };
+    lds_double* const ys = (lds_double*)&ysc[wave][0];
+    uint32_t wc[U];
+    double wy[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) entry(rel + (uint32_t)(lane + kWave * j), wc[j], wy[j]);
+    for (int s = (int)uni(s0[wave]); s < k && dd.fid[s] < ce; ++s) {
       const uint32_t lo = max(dd.fid[s], cb), hi = min(dd.fid[s + 1], ce);
       begin(s);
-      for (uint32_t base = lo; base < hi; base += kWave) {
-        const uint32_t lim = min(base + (uint32_t)kWave, hi);
-        const uint32_t cl = base + (uint32_t)lane;
-        double xrow[R];
-        load_row<double, R>(rows, cl < lim ? cl : lim - 1, xrow);
-        // the window's entries inside [base, lim) form a prefix of it
-        const bool in = wc < lim;
-        const int m = popc64(wave_ballot(in));
-        ysc[wave][lane] = 0.0;
-        wave_lds_sync();
-        if (in) ysc[wave][wc - base] = wy;
-        wave_lds_sync();
-        const double y = ysc[wave][lane];
-        ip += (uint64_t)m;
-        e = ip + (uint64_t)lane;
-        ec = e < g1 ? e : e_last;
-        wc = gcell[ec];
-        wy = gy[ec];
-        wc = e < g1 ? wc : 0xffffffffu;
-        if (cl < lim) cell(s, cl, y, xrow);
+      for (uint32_t base = lo; base < hi; base += TILE) {
+        const uint32_t tend = min(base + TILE, hi);
+        double xr[U][R];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint32_t cl = base + (uint32_t)(lane + kWave * u);
+          load_row_at<double, R>(rows, cl < tend ? cl : tend - 1, xr[u]);
+        }
+        // the window's entries inside [base, tend) form a prefix of it
+        uint32_t m = 0;
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+          const bool in = wc[j] < tend;
+          if (in) ys[wc[j] - base] = wy[j];
+          m += (uint32_t)popc64(wave_ballot(in));
+        }
+        rel += m;
+#pragma unroll
+        for (int j = 0; j < U; ++j) entry(rel + (uint32_t)(lane + kWave * j), wc[j], wy[j]);
+        __builtin_amdgcn_wave_barrier();
+        double yv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) yv[u] = ys[lane + kWave * u];
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int u = 0; u < U; ++u) ys[lane + kWave * u] = 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint32_t cl = base + (uint32_t)(lane + kWave * u);
+          if (cl < tend) cell(s, cl, yv[u], xr[u]);
+        }
       }
       flush(s);
     }
@@ -212,7 +243,7 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_pass_kernel(DeviceData dd, co
                                                                int chunks, int n_req) {
   using D = PtmgDims<NB>;
   constexpr int PS = D::PS, PG = D::PG, NTRI = D::NTRI;
-  __shared__ double ysc[WAVES][kWave];
+  __shared__ double ysc[WAVES][kWave * kPtmgTile];
   __shared__ uint64_t ip0[WAVES];
   __shared__ uint32_t s0[WAVES];
   __shared__ double f64tab[kF64TabDoubles];  // the FP64 sweep's exp / log tables
@@ -260,19 +291,16 @@ __global__ __launch_bounds__(WAVES * 64) void ptmg_pass_kernel(DeviceData dd, co
   // eta = log offset + x'beta from a packed row (non-intercept columns, then
   // the log offset), columns added in design order as x64 holds them
   const int icpt = (int)dd.icpt;
+  // (stored column j before the intercept, j - 1 after it: constant indices
+  // and a select, so the row stays in registers)
   auto row_eta = [&](const double* prow, double xr[NB]) {
     double eta = prow[NB - 1];
-    int c = 0;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
-      if (j == icpt) {
-        xr[j] = 1.0;
-        eta += beta[j];
-      } else {
-        xr[j] = prow[c];
-        eta += prow[c] * beta[j];
-        ++c;
-      }
+      const double before = j < NB - 1 ? prow[j] : 0.0;
+      const double after = j > 0 ? prow[j - 1] : 0.0;
+      xr[j] = j < icpt ? before : (j == icpt ? 1.0 : after);
+      eta += xr[j] * beta[j];
     }
     return eta;
   };
