@@ -53,7 +53,7 @@ namespace nh {
 
 struct DeviceData {
   const float* row32;    // n_cells * rw: non-intercept design columns, log offset, padding
-  const double* row64;   // the same in f64
+  const double* row64;   // the same in f64, n_cells * rw64
   const double* x64;     // n_cells * nb centred design (stage one)
   const double* off64;   // log offset (stage one)
   const uint32_t* fid;   // k + 1 subject boundaries
@@ -70,7 +70,7 @@ struct DeviceData {
   const uint32_t* zsl;
   const uint32_t* zindex;
   const double* zcoef;
-  uint32_t n_cells, nb, k, icpt, rw, tabled;
+  uint32_t n_cells, nb, k, icpt, rw, rw64, tabled;
 };
 
 struct FitKnobs {
@@ -134,7 +134,13 @@ constexpr int kStageFloats =
 template <int NB>
 struct Dims {
   static constexpr int Q = NB - 1;                // stored design columns
+  // values per packed row: f32 rows padded to 1, 2, 4 or 8 (float4 loads),
+  // f64 rows to the next even count (double2 loads), so p = 5, 6 read 48
+  // bytes per cell, not 64
   static constexpr int R = Q + 1 <= 1 ? 1 : (Q + 1 <= 2 ? 2 : (Q + 1 <= 4 ? 4 : 8));
+  static constexpr int R64 = Q + 1 <= 1 ? 1 : ((Q + 2) & ~1);
+  template <class CT>
+  static constexpr int RW = sizeof(CT) == 4 ? R : R64;
   static constexpr int QT = Q * (Q + 1) / 2;
   static constexpr int NS = 2 + Q;                // per subject: resid, weight, first[Q]
   static constexpr int NF = 4 + Q;                // SW_FINISH: resid, weight, dweight, dresid/dgamma,
@@ -155,6 +161,10 @@ constexpr int kWavesPerEu = NB <= 4 ? 4 : (NB <= 6 ? 3 : 2);
 __host__ __device__ inline uint32_t packed_row_width(uint32_t nb) {
   const uint32_t r = nb;  // nb - 1 columns + offset
   return r <= 1 ? 1u : (r <= 2 ? 2u : (r <= 4 ? 4u : 8u));
+}
+// the f64 rows' width (Dims::R64)
+__host__ __device__ inline uint32_t packed_row_width64(uint32_t nb) {
+  return nb <= 1 ? 1u : ((nb + 1u) & ~1u);
 }
 
 // Per-fit work arrays: k-sized and partial-sum buffers, in LDS when they fit.
@@ -518,7 +528,7 @@ __device__ __forceinline__ void sweep_wave(const CT* __restrict__ rows,
                                         int s, int k, uint32_t cb, uint32_t ce, int icpt,
                                         double gamma, int lane, int wave) {
   using D = Dims<NB>;
-  constexpr int Q = D::Q, R = D::R, NS = D::NS, NSL = D::NSL, NG = D::NG;
+  constexpr int Q = D::Q, R = D::template RW<CT>, NS = D::NS, NSL = D::NSL, NG = D::NG;
   constexpr int U = tile_cells<CT, MODE>();
   // every argument but lane is the wave's own: scalar registers
   gconst_char* const grows = (gconst_char*)uni(rows);
@@ -761,7 +771,7 @@ __device__ __forceinline__ void sweep_wave_tabled(
     double* __restrict__ psub, double* __restrict__ pgene, uint64_t ip, uint64_t g1, int s, int k,
     uint32_t cb, uint32_t ce, int icpt, double gamma, int lane, int wave) {
   using D = Dims<NB>;
-  constexpr int Q = D::Q, R = D::R, NS = D::NS, NSL = D::NSL, NG = D::NG;
+  constexpr int Q = D::Q, R = D::R64, NS = D::NS, NSL = D::NSL, NG = D::NG;
   constexpr int NSV = MODE == SW_LAPLACE ? 4 : (MODE == SW_FINISH ? D::NF : NS);
   constexpr int NGV = MODE == SW_LAPLACE ? 0 : (MODE == SW_FINISH ? D::NGF : NG);
   constexpr int QA = Q > 0 ? Q : 1;

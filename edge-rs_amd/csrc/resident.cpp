@@ -42,18 +42,19 @@ void Resident::upload(size_t nc, size_t nb_, size_t k_, const double* design,
   if (icpt == nb)
     throw std::invalid_argument("the design needs an intercept column (1 in every cell)");
   rw = packed_row_width((uint32_t)nb);
+  rw64 = packed_row_width64((uint32_t)nb);
   std::vector<float> h32(n_cells * rw, 0.0f);
-  std::vector<double> h64(n_cells * rw, 0.0);
+  std::vector<double> h64(n_cells * rw64, 0.0);
   for (size_t i = 0; i < n_cells; ++i) {
     size_t q = 0;
     for (size_t j = 0; j < nb; ++j) {
       if (j == icpt) continue;
       h32[i * rw + q] = (float)design[i * nb + j];
-      h64[i * rw + q] = design[i * nb + j];
+      h64[i * rw64 + q] = design[i * nb + j];
       ++q;
     }
     h32[i * rw + q] = (float)log_offset[i];
-    h64[i * rw + q] = log_offset[i];
+    h64[i * rw64 + q] = log_offset[i];
   }
   row32.upload(h32.data(), h32.size(), stream);
   row64.upload(h64.data(), h64.size(), stream);
@@ -123,6 +124,7 @@ DeviceData Resident::view() const {
   d.k = (uint32_t)k;
   d.icpt = (uint32_t)icpt;
   d.rw = rw;
+  d.rw64 = rw64;
   d.tabled = tabled ? 1u : 0u;
   d.zgroup = zgroup.p;
   d.zsg = zsg.p;

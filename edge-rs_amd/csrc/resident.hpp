@@ -89,7 +89,8 @@ struct Resident {
   int device = 0;
   hipStream_t stream = nullptr;
   size_t n_cells = 0, nb = 0, k = 0, n_genes = 0, icpt = 0;
-  uint32_t rw = 1;  // values per packed cell row
+  uint32_t rw = 1;    // values per packed f32 cell row
+  uint32_t rw64 = 1;  // values per packed f64 cell row
   DevBuf<float> row32, gy32;
   DevBuf<double> row64;
   DevBuf<double> x64, off64, gy64, gtot;

@@ -160,7 +160,7 @@ struct PtmgWalk {
   // what it read.
   template <class Begin, class Cell, class Flush>
   __device__ void walk(Begin&& begin, Cell&& cell, Flush&& flush) {
-    constexpr int R = Dims<NB>::R;
+    constexpr int R = Dims<NB>::R64;
     constexpr int U = kPtmgTile;
     constexpr uint32_t TILE = kWave * U;
     const uint32_t cb = bb(), ce = be();
