@@ -51,11 +51,38 @@ enum BobyqaStatus : int {
 #define NH_STRICT_FP
 #endif
 
+// Local n, npt and ndim, hiding the members of the same names: compile-time
+// constants when the dimension is fixed.  On the device, an IN_LDS object's
+// address is declared to be in workgroup memory.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define NH_BQ_WHERE \
+  if constexpr (IN_LDS) __builtin_assume(__builtin_amdgcn_is_shared((const void*)this));
+#else
+#define NH_BQ_WHERE
+#endif
+#define NH_BQ_DIMS                                                   \
+  NH_BQ_WHERE                                                        \
+  [[maybe_unused]] const int n = kFix ? NFIX : this->n;              \
+  [[maybe_unused]] const int npt = kFix ? 2 * NFIX + 1 : this->npt;  \
+  [[maybe_unused]] const int ndim = kFix ? 3 * NFIX + 1 : this->ndim;
+
 NH_HD inline bool nh_finite(double v) { return __builtin_isfinite(v); }
 NH_HD inline bool nh_inf(double v) { return __builtin_isinf(v); }
 
-template <int NMAX>
+// NFIX > 0 fixes the dimension at compile time (n = NFIX, npt = 2n + 1,
+// ndim = npt + n; init refuses any other n): every loop over the coordinates
+// and the interpolation points then has a constant trip count, so the device
+// compiler unrolls it and addresses Powell's arrays at constant offsets
+// instead of running loop counters and 64-bit address arithmetic on one lane.
+// The arithmetic and its order are the same as with a run-time n.
+//
+// IN_LDS: on the device the object lives in workgroup memory (LDS), so its
+// members are read and written with ds_ instructions at 32-bit addresses
+// rather than through flat (generic) pointers.
+template <int NMAX, int NFIX = 0, bool IN_LDS = false>
 struct Bobyqa {
+  static_assert(NFIX == 0 || NFIX == NMAX, "a fixed dimension is NMAX");
+  static constexpr bool kFix = NFIX > 0;
   static constexpr int NPTMAX = 2 * NMAX + 1;
   static constexpr int NDIMMAX = NPTMAX + NMAX;
   static constexpr int NHMAX = NMAX * (NMAX + 1) / 2;
@@ -103,9 +130,18 @@ struct Bobyqa {
   int kpt, rip, riq;
   double rvquad, fbase, rxp, rxq;
 
-  NH_HD int ix(int k, int j) const { return (k - 1) + (j - 1) * npt; }
-  NH_HD int ib(int k, int j) const { return (k - 1) + (j - 1) * ndim; }
-  NH_HD int iz(int k, int j) const { return (k - 1) + (j - 1) * npt; }
+  NH_HD int ix(int k, int j) const {
+    NH_BQ_DIMS
+    return (k - 1) + (j - 1) * npt;
+  }
+  NH_HD int ib(int k, int j) const {
+    NH_BQ_DIMS
+    return (k - 1) + (j - 1) * ndim;
+  }
+  NH_HD int iz(int k, int j) const {
+    NH_BQ_DIMS
+    return (k - 1) + (j - 1) * npt;
+  }
 
   // ---------------------------------------------------------------------
   // Set-up: nlopt_set_default_initial_step, nlopt_compute_rescaling and the
@@ -149,6 +185,10 @@ struct Bobyqa {
     want = false;
     status = BOBYQA_RUNNING;
     fbest_out = INFINITY;
+    if (kFix && n_ != NFIX) {
+      status = BOBYQA_BAD_BOX;
+      return;
+    }
     for (int i = 0; i <= n; ++i) s[i] = 1.0;
     bool differ = false;
     for (int i = 1; i < n; ++i) differ = differ || (dx[i] != dx[i - 1]);
@@ -243,6 +283,7 @@ struct Bobyqa {
 
   NH_HD void ask_point(int where) {
     NH_STRICT_FP
+    NH_BQ_DIMS
     for (int j = 1; j <= n; ++j) xask[j - 1] = x[j] * s[j];
     ++nevals;
     resume = where;
@@ -251,6 +292,7 @@ struct Bobyqa {
 
   NH_HD void finish_at_best() {
     NH_STRICT_FP
+    NH_BQ_DIMS
     for (int i = 1; i <= n; ++i) xbest_out[i - 1] = x[i] * s[i];
   }
 
@@ -260,6 +302,7 @@ struct Bobyqa {
   // The resumable body of prelim + bobyqb + rescue.
   NH_HD void advance() {
     NH_STRICT_FP
+    NH_BQ_DIMS
     switch (resume) {
       case R_START: goto L_prelim_next;
       case R_PRELIM: goto L_prelim_post;
@@ -664,6 +707,7 @@ struct Bobyqa {
   // Label 90 body: move xbase to xopt (bobyqa.rs:904-980).
   NH_HD void shift_base() {
     NH_STRICT_FP
+    NH_BQ_DIMS
     const int nptm = npt - n - 1;
     double fracsq = xoptsq * 0.25;
     double sumpq = 0.0;
@@ -729,6 +773,7 @@ struct Bobyqa {
   // stop (status set).
   NH_HD bool model_update() {
     NH_STRICT_FP
+    NH_BQ_DIMS
     const int nptm = npt - n - 1;
     const int nh = n * (n + 1) / 2;
     fopt = fval[kopt];
@@ -911,10 +956,17 @@ struct Bobyqa {
 
   // ---- rescue (bobyqa.rs:1540-1932), split around its evaluations ----
   NH_HD int pa(int r, int j) const { return 1 + (r - 1) + 2 * (j - 1); }
-  NH_HD int pid(int k) const { return 2 * n + k; }
-  NH_HD int rw(int i) const { return ndim + n + i; }
+  NH_HD int pid(int k) const {
+    NH_BQ_DIMS
+    return 2 * n + k;
+  }
+  NH_HD int rw(int i) const {
+    NH_BQ_DIMS
+    return ndim + n + i;
+  }
   NH_HD void pid_parts(int k, int& ip, int& iq) const {
     NH_STRICT_FP
+    NH_BQ_DIMS
     double np = (double)(n + 1);
     double v = w[pid(k)];
     long a = (long)v;
@@ -927,6 +979,7 @@ struct Bobyqa {
   // (Powell's NREM = 0 exit to label 350), true to go on to label 260.
   NH_HD bool rescue_prepare() {
     NH_STRICT_FP
+    NH_BQ_DIMS
     const int np = n + 1;
     const int nptm = npt - np;
     const double sfrac = 0.5 / (double)np;
@@ -1111,6 +1164,7 @@ struct Bobyqa {
   // Label 260: place provisional point kpt and its model value.
   NH_HD void rescue_point() {
     NH_STRICT_FP
+    NH_BQ_DIMS
     int ih = 0;
     for (int j = 1; j <= n; ++j) {
       w[rw(j)] = xpt[ix(kpt, j)];
@@ -1166,6 +1220,7 @@ struct Bobyqa {
 
   NH_HD void rescue_update() {
     NH_STRICT_FP
+    NH_BQ_DIMS
     const int nptm = npt - n - 1;
     double diff = f - rvquad;
     for (int i = 1; i <= n; ++i) gopt[i] += diff * bmat[ib(kpt, i)];
@@ -1202,6 +1257,7 @@ struct Bobyqa {
   NH_HD void update_hz(int kn) { update_with(beta, denom, kn, 1); }
   NH_HD void update_with(double ubeta, double udenom, int kn, int wbase) {
     NH_STRICT_FP
+    NH_BQ_DIMS
     double* wk = w + wbase;
     double* vl = vlag + 1;
     const int nptm = npt - n - 1;
@@ -1245,6 +1301,7 @@ struct Bobyqa {
   // ---- TRSBOX (bobyqa.rs:2007-2432) with Powell's labels ----
   NH_HD void trsbox_step() {
     NH_STRICT_FP
+    NH_BQ_DIMS
     double* gnew = w + 1;
     double* xbdi = w + 1 + n;
     double* sv = w + 1 + 2 * n;
@@ -1524,6 +1581,7 @@ struct Bobyqa {
   // ---- ALTMOV (bobyqa.rs:2438-2708) ----
   NH_HD void altmov_step() {
     NH_STRICT_FP
+    NH_BQ_DIMS
     double* glag = w + 1;
     double* hcol = w + 1 + n;
     double* wk = w + ndim + 1;

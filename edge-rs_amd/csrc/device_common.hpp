@@ -47,6 +47,15 @@ __device__ __forceinline__ T* uni(T* p) {
 
 __device__ __forceinline__ int popc64(uint64_t m) { return __popcll(m); }
 
+// Declares that p points into workgroup memory (LDS), so the compiler
+// addresses what it reaches through p with ds_ instructions at 32-bit
+// addresses instead of flat ones.  Only for pointers that do point there.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define NH_ASSUME_LDS(p) __builtin_assume(__builtin_amdgcn_is_shared((const void*)(p)))
+#else
+#define NH_ASSUME_LDS(p) ((void)0)
+#endif
+
 // Makes this wave's LDS writes visible to its other lanes before they read.
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -69,14 +69,20 @@ __device__ static double device_objective(double subject, double cell, const Fit
 }
 
 // The search control runs on one lane between fits; kept out of line so the
-// sweep loop's register allocation is not shaped by it.
+// sweep loop's register allocation is not shaped by it.  Its state is the
+// kernels' __shared__ SearchControl (ds_ accesses, not flat).
 __device__ __noinline__ static void ctl_begin(SearchControl* c, const DevSearch* S,
                                               const float* pred, const PersistentArgs* A) {
+  NH_ASSUME_LDS(c);
   c->begin((int)S->kind, (int)S->ord, S->start, S->kind == kSearchBoth ? pred : nullptr, A->lo,
            A->hi, A->warm_step, A->warm_xtol);
 }
-__device__ __noinline__ static int ctl_tell(SearchControl* c, double v) { return c->tell(v); }
+__device__ __noinline__ static int ctl_tell(SearchControl* c, double v) {
+  NH_ASSUME_LDS(c);
+  return c->tell(v);
+}
 __device__ __noinline__ static void ctl_out(const SearchControl* c, DevSearchOut* o) {
+  NH_ASSUME_LDS(c);
   double f;
   int ord;
   c->result(o->x, f, ord);

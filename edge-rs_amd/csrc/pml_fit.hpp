@@ -922,6 +922,10 @@ __device__ __forceinline__ void sweep_wave_tabled(
 
 // ---------------------------------------------------------------------------
 // The fit of one request, all threads of the workgroup cooperating.
+// Every fit kernel holds its FitLds in __shared__ memory: the out-of-line fit
+// steps declare so, and address it with ds_ instructions rather than flat.
+#define NH_FIT_LDS NH_ASSUME_LDS(&sh);
+
 template <int NB, int WAVES>
 struct Fitter {
   using D = Dims<NB>;
@@ -971,6 +975,7 @@ struct Fitter {
   // search over the gene's sorted cells) and the subject holding it; clears
   // the wave's count staging row.
   __device__ __noinline__ void setup() {
+    NH_FIT_LDS
     uint32_t cb = block_begin(wave);
     uint64_t lo = g0, hi = g1;
     while (hi - lo > (uint64_t)kWave) {
@@ -1019,6 +1024,7 @@ struct Fitter {
 
   template <class CT, int MODE>
   __device__ __noinline__ void sweep_dense(const double* bsrc, const double* lwsrc) {
+    NH_FIT_LDS
     const CT* rows = sizeof(CT) == 4 ? (const CT*)dd.row32 : (const CT*)dd.row64;
     const CT* gy = sizeof(CT) == 4 ? (const CT*)dd.gy32 : (const CT*)dd.gy64;
     sweep_wave<CT, MODE, NB>(rows, dd.gcell, gy, dd.fid, bsrc, lwsrc,
@@ -1033,6 +1039,7 @@ struct Fitter {
   // lookups and the positive-count increments), counted as a sweep of CT.
   template <class CT, int MODE>
   __device__ __noinline__ void sweep_tabled(const double* bsrc, const double* lwsrc) {
+    NH_FIT_LDS
     sweep_wave_tabled<MODE, NB>(dd.row64, dd.gcell, dd.gy64, dd.fid, dd.zgroup, dd.zsg, dd.zloose,
                                 dd.zsl, dd.zindex, dd.zcoef, bsrc, lwsrc,
                                 (const lds_double*)&sh.f64tab[0], wk.psub, wk.pgene,
@@ -1047,6 +1054,7 @@ struct Fitter {
   // cells, the exact optimum when every cell's exp(eta) is small against
   // gamma; the Newton loop starts there instead of at log w = 0.
   __device__ __noinline__ void mean_start() {
+    NH_FIT_LDS
     sweep_wave_mean<NB>(dd.row32, dd.fid, sh.beta, wk.psub, (int)sh.s0[wave], k,
                         block_begin(wave), block_end(wave), icpt, lane, wave);
     __syncthreads();
@@ -1119,6 +1127,7 @@ struct Fitter {
   __device__ __noinline__ void finalize(int sd, const double* lwsrc,
                                         const double* hvwb = nullptr, int hstride = 0,
                                         const double* hvb = nullptr) {
+    NH_FIT_LDS
     const double* tot_y = dd.gtot + (size_t)gene * k;
     double* sub = side(sd);
     const double* hsub = hvwb;
@@ -1185,6 +1194,7 @@ struct Fitter {
   // Schur complement of side sd into sh.vb2 (and the right-hand side into
   // sh.rhs when `with_rhs`), one wave per entry.
   __device__ __noinline__ void schur(int sd, bool with_rhs) {
+    NH_FIT_LDS
     const double* sub = side(sd);
     const int n_e = NTRI + (with_rhs ? NB : 0);
     for (int e = wave; e < n_e; e += WAVES) {
@@ -1217,6 +1227,7 @@ struct Fitter {
   // Newton step from side `sd` (pml.rs:1664-1710): Schur complement, LDL'
   // solve, per-subject steps; writes sh.new_beta and wk.new_lw.
   __device__ __noinline__ void newton_step(int sd) {
+    NH_FIT_LDS
     const double* sub = side(sd);
     schur(sd, true);
     if (tid == 0) {
@@ -1250,6 +1261,7 @@ struct Fitter {
   // NaN when a coordinate moves by more than 0.1, where the model is not
   // trusted.
   __device__ __noinline__ double decrement(int sd) {
+    NH_FIT_LDS
     const double* sub = side(sd);
     if (wave == 0) {
       double acc = 0.0, big = 0.0;
@@ -1274,6 +1286,7 @@ struct Fitter {
 
   // Per-coordinate damping of the trial point (pml.rs:1729-1755).
   __device__ __noinline__ void damp(double min_step) {
+    NH_FIT_LDS
     const double cut = 40.0;
     if (tid < NB) {
       const int j = tid;
@@ -1302,6 +1315,7 @@ struct Fitter {
   }
 
   __device__ __noinline__ void accept() {
+    NH_FIT_LDS
     if (tid < NB) sh.beta[tid] = sh.new_beta[tid];
     for (int s = tid; s < k; s += NT) wk.lw[s] = wk.new_lw[s];
     __syncthreads();
@@ -1309,6 +1323,7 @@ struct Fitter {
 
   // max |gradient| over db and dw of side sd (pml.rs:1720-1724)
   __device__ __noinline__ double max_grad(int sd) {
+    NH_FIT_LDS
     if (tid == 0) {
       double m = 0.0;
       for (int j = 0; j < NB; ++j) m = fmax(m, fabs(sh.db[sd][j]));
@@ -1398,6 +1413,7 @@ struct Fitter {
   // sum_s log |vw_s| at the point newton_step(sd) stepped to, from side sd's
   // vw and an SW_FINISH sweep's eta-derivatives of the weight sums.
   __device__ __noinline__ double stepped_log_det(int sd) {
+    NH_FIT_LDS
     const double* sub = side(sd);
     if (wave == 0) {
       const double acc = subject_sum([&](int s) {
@@ -1492,6 +1508,7 @@ struct Fitter {
   // d resid / d gamma sums).  jb: 2 x NB (LDS), jl: 2 x k.  The same Schur
   // complement and pivoted LDL' solve as newton_step.
   __device__ __noinline__ void sensitivity(int sd, double subject, double* jb, double* jl) {
+    NH_FIT_LDS
     const double* sub = side(sd);
     const double exps = exp(subject), em1 = exps - 1.0;
     const double dalpha = -exps / (em1 * em1);
@@ -1538,6 +1555,7 @@ struct Fitter {
 
   // log-det from the last Newton step's vw (pml.rs:1767-1771)
   __device__ __noinline__ double log_det_step() {
+    NH_FIT_LDS
     if (tid == 0) {
       double ld = 0.0;
       for (int s = 0; s < k; ++s) ld += log(fabs(wk.vw_step[s]));
@@ -1549,6 +1567,7 @@ struct Fitter {
 
   // Laplace correction at the current point (pml.rs:1828-1919), f64.
   __device__ __noinline__ double laplace(unsigned ord) {
+    NH_FIT_LDS
     sweep<double, SW_LAPLACE>(sh.beta, wk.lw);
     if (tid == 0) {
       const bool high = ord > 2;

@@ -46,7 +46,7 @@ struct SearchControl {
   double wx[2], wf;           // the warm pass's result
   int wfailed;
   double vals[3];
-  Bobyqa<2> bq;
+  Bobyqa<2, 2, true> bq;  // on the device a SearchControl lives in LDS
   SubjectSearch ss;
 
   NH_HD static double clampd(double v, double a, double b) { return v < a ? a : (v > b ? b : v); }
