@@ -43,6 +43,10 @@ void Resident::upload(size_t nc, size_t nb_, size_t k_, const double* design,
     throw std::invalid_argument("the design needs an intercept column (1 in every cell)");
   rw = packed_row_width((uint32_t)nb);
   rw64 = packed_row_width64((uint32_t)nb);
+  // the sweeps address a cell's f64 row at a 32-bit byte offset from the
+  // array's base
+  if (n_cells * rw64 * sizeof(double) > 0xffffffffull)
+    throw std::invalid_argument("too many cells for 32-bit row offsets at this design width");
   std::vector<float> h32(n_cells * rw, 0.0f);
   std::vector<double> h64(n_cells * rw64, 0.0);
   for (size_t i = 0; i < n_cells; ++i) {
