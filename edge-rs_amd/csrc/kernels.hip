@@ -22,6 +22,7 @@ void pml_batch_kernel(
     DevReply* out, double* out_beta, double* out_lw, double* out_info, double* scratch,
     size_t stride, FitKnobs kn, int mode, FitCounters* ctr, int work_in_lds) {
   __shared__ FitLds<NB, WAVES> sh;
+  __shared__ Fitter<NB, WAVES> f;
   extern __shared__ double nh_batch_dyn[];
   const int q = blockIdx.x;
   const int k = (int)dd.k;
@@ -31,11 +32,13 @@ void pml_batch_kernel(
   if (threadIdx.x < NB) sh.beta[threadIdx.x] = beta_init[(size_t)q * NB + threadIdx.x];
   for (int s = threadIdx.x; s < k; s += WAVES * 64)
     wk.lw[s] = lw_init ? lw_init[(size_t)q * k + s] : 0.0;
+  if (threadIdx.x == 0) {
+    const double exps = exp(r.subject);
+    const double alpha = 1.0 / (exps - 1.0);
+    const double lambda = 1.0 / (sqrt(exps) * (exps - 1.0));
+    f.init(dd, &sh, wk, ctr, r.gene, r.cell, alpha, lambda);
+  }
   __syncthreads();
-  const double exps = exp(r.subject);
-  const double alpha = 1.0 / (exps - 1.0);
-  const double lambda = 1.0 / (sqrt(exps) * (exps - 1.0));
-  Fitter<NB, WAVES> f(dd, sh, wk, ctr, r.gene, r.cell, alpha, lambda);
   FitResult res = f.run(mode, r.ord & ~kOrdMeanStart, kn, false, (r.ord & kOrdMeanStart) != 0);
   if (threadIdx.x == 0) {
     DevReply o;

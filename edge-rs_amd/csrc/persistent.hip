@@ -99,6 +99,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(kWav
 void stage_two_kernel(PersistentArgs A) {
   constexpr int NT = WAVES * 64;
   __shared__ FitLds<NB, WAVES> sh;
+  __shared__ Fitter<NB, WAVES> f;  // the gene's fit object
   __shared__ SearchControl ctl;
   __shared__ double warm_beta[NB];
   __shared__ double warm_jb[2 * NB];  // d beta* / d (subject, gamma) at the anchor
@@ -124,18 +125,26 @@ void stage_two_kernel(PersistentArgs A) {
     if (slot >= A.n_search) break;
     const DevSearch S = A.search[A.order[slot]];
     if (tid == 0) s_pred = 0;
-    if (tid == 0)
+    if (tid == 0) {
       ctl_begin(&ctl, &A.search[A.order[slot]],
                 A.pred ? A.pred + 2 * (size_t)A.order[slot] : nullptr, &A);
+      // one fit object per gene: members set here, the variance components
+      // once per evaluation
+      f.init(dd, &sh, wk, A.counters, S.gene, 0.0, 0.0, 0.0);
+    }
     __syncthreads();
-    // one fit object per gene (it lives in private memory: its members are
-    // written once here, the variance components once per evaluation)
-    Fitter<NB, WAVES> f(dd, sh, wk, A.counters, S.gene, 0.0, 0.0, 0.0);
     while (ctl.active) {
       if (tid == 0) ctl.point(s_subject, s_cell);
       __syncthreads();
       const double subject = s_subject, cell = s_cell;
       const int have_warm = ctl.have_warm;
+      if (tid == 0) {
+        // published by the barrier after the start values
+        const double exps = exp(subject);
+        f.gamma = cell;
+        f.alpha = 1.0 / (exps - 1.0);
+        f.lambda = 1.0 / (sqrt(exps) * (exps - 1.0));
+      }
       // A warm fit starts from the anchor fit's optimum moved to first order
       // to this (subject, gamma), unless some coordinate would move by more
       // than kPredictCap (where the linear model is not trusted).
@@ -160,12 +169,6 @@ void stage_two_kernel(PersistentArgs A) {
                                                   : 0.0)
                              : 0.0;
       __syncthreads();
-      const double exps = exp(subject);
-      const double alpha = 1.0 / (exps - 1.0);
-      const double lambda = 1.0 / (sqrt(exps) * (exps - 1.0));
-      f.gamma = cell;
-      f.alpha = alpha;
-      f.lambda = lambda;
       // a predicted start goes straight to the FP64 finish, with the anchor
       // fit's curvature blocks
       const FitResult r = f.run(0, (unsigned)ctl.point_order(), A.knobs, ctl.evals > 0,

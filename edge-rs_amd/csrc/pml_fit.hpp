@@ -922,9 +922,17 @@ __device__ __forceinline__ void sweep_wave_tabled(
 
 // ---------------------------------------------------------------------------
 // The fit of one request, all threads of the workgroup cooperating.
-// Every fit kernel holds its FitLds in __shared__ memory: the out-of-line fit
-// steps declare so, and address it with ds_ instructions rather than flat.
-#define NH_FIT_LDS NH_ASSUME_LDS(&sh);
+// Every fit kernel holds its fit object and its FitLds in __shared__ memory:
+// each step of the fit declares so (members and LDS arrays then read with
+// ds_ instructions, not flat ones through private memory), binds `sh` to the
+// kernel's FitLds and takes its thread indices from the hardware.
+#define NH_FIT_LDS                                 \
+  NH_ASSUME_LDS(this);                             \
+  [[maybe_unused]] FitLds<NB, WAVES>& sh = *shp;   \
+  NH_ASSUME_LDS(&sh);                              \
+  [[maybe_unused]] const int tid = (int)threadIdx.x; \
+  [[maybe_unused]] const int wave = tid / kWave;   \
+  [[maybe_unused]] const int lane = tid % kWave;
 
 template <int NB, int WAVES>
 struct Fitter {
@@ -933,29 +941,35 @@ struct Fitter {
   static constexpr int Q = D::Q, R = D::R, QT = D::QT, NS = D::NS, NSL = D::NSL, NG = D::NG;
   static constexpr int SUB = D::SUB, NTRI = D::NTRI;
 
-  const DeviceData& dd;
-  FitLds<NB, WAVES>& sh;
-  FitWork<NB>& wk;
+  // The fit object is __shared__ in every fit kernel (see NH_FIT_LDS).
+  DeviceData dd;
+  FitLds<NB, WAVES>* shp;
+  FitWork<NB> wk;
   FitCounters* counters;
   uint32_t gene;
   uint64_t g0, g1;   // CSR range
   double gamma, alpha, lambda;
   int k, icpt;
   uint32_t blk;      // cells per wave block
-  int tid, wave, lane;
 
-  __device__ Fitter(const DeviceData& d, FitLds<NB, WAVES>& s, FitWork<NB>& w,
-                    FitCounters* ctr, uint32_t g, double gam, double alp, double lam)
-      : dd(d), sh(s), wk(w), counters(ctr), gene(g), gamma(gam), alpha(alp), lambda(lam) {
-    k = (int)dd.k;
-    icpt = (int)dd.icpt;
-    g0 = dd.gptr[g];
-    g1 = dd.gptr[g + 1];
-    uint32_t per = (dd.n_cells + WAVES - 1) / WAVES;
+  // Set-up for gene g by one thread; the workgroup synchronises before the
+  // fit starts.
+  __device__ void init(const DeviceData& d, FitLds<NB, WAVES>* s, const FitWork<NB>& w,
+                       FitCounters* ctr, uint32_t g, double gam, double alp, double lam) {
+    dd = d;
+    shp = s;
+    wk = w;
+    counters = ctr;
+    gene = g;
+    gamma = gam;
+    alpha = alp;
+    lambda = lam;
+    k = (int)d.k;
+    icpt = (int)d.icpt;
+    g0 = d.gptr[g];
+    g1 = d.gptr[g + 1];
+    const uint32_t per = (d.n_cells + WAVES - 1) / WAVES;
     blk = (per + kWave - 1) / kWave * kWave;
-    tid = threadIdx.x;
-    wave = tid / kWave;
-    lane = tid % kWave;
   }
 
   __device__ double* side(int sd) const { return wk.subs + (size_t)sd * k * SUB; }
@@ -963,7 +977,7 @@ struct Fitter {
   // the FP64 exp / log tables (filled by setup); the per-subject steps use
   // the sweeps' exp and log, whose short code keeps these steps' register
   // use, and so their call-site saves, small
-  __device__ const lds_double* tab() const { return (const lds_double*)&sh.f64tab[0]; }
+  __device__ const lds_double* tab() const { return (const lds_double*)&shp->f64tab[0]; }
 
   // design column of stored column q
   __device__ static constexpr int qcol(int q, int ic) { return q < ic ? q : q + 1; }
@@ -1016,6 +1030,7 @@ struct Fitter {
   // that neither's registers are allocated around a call to the other.
   template <class CT, int MODE>
   __device__ __forceinline__ void sweep(const double* bsrc, const double* lwsrc) {
+    NH_FIT_LDS
     if (dd.tabled)
       sweep_tabled<CT, MODE>(bsrc, lwsrc);
     else
@@ -1075,6 +1090,7 @@ struct Fitter {
   // per-gene totals likewise; the sweep counters.
   template <class CT, int MODE>
   __device__ __forceinline__ void sweep_totals() {
+    NH_FIT_LDS
     constexpr int NSV = MODE == SW_LAPLACE ? 4 : (MODE == SW_FINISH ? D::NF : NS);
     constexpr int NGV = MODE == SW_LAPLACE ? 0 : (MODE == SW_FINISH ? D::NGF : NG);
     __syncthreads();
@@ -1111,6 +1127,7 @@ struct Fitter {
   // l, l + 64, ...; the wave's fixed butterfly adds the lanes.
   template <class F>
   __device__ double subject_sum(F f) const {
+    NH_FIT_LDS
     double acc = 0.0;
     for (int s = lane; s < k; s += kWave) acc += f(s);
     return wave_sum(acc);
@@ -1342,6 +1359,7 @@ struct Fitter {
   // final point; out gets ll, ll_prev, iterations, backtracks.
   template <class CT>
   __device__ int newton_loop(const FitKnobs& kn, double noise_rel, FitResult& out) {
+    NH_FIT_LDS
     int cur = 0;
     sweep<CT, SW_FULL>(sh.beta, wk.lw);
     finalize(cur, wk.lw);
@@ -1440,6 +1458,7 @@ struct Fitter {
   // the budget runs out.
   __device__ bool finish(const FitKnobs& kn, FitResult& out, int cur, const double* hvwb,
                          int hstride, const double* hvb) {
+    NH_FIT_LDS
     if (kn.decrement_stop) {
       sweep<double, SW_FINISH>(sh.beta, wk.lw);
       finalize(cur, wk.lw, hvwb, hstride, hvb);
@@ -1613,6 +1632,7 @@ struct Fitter {
   __device__ FitResult run(int mode, unsigned ord, const FitKnobs& kn, bool same_gene = false,
                            bool cold = false, const double* near_vwb = nullptr,
                            const double* near_vb = nullptr) {
+    NH_FIT_LDS
     FitResult r{};
     r.finish_steps = 0;
     r.exact = 0;
